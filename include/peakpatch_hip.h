@@ -1,0 +1,198 @@
+/*
+ * peakpatch_hip.h -- C ABI of libpeakpatch_hip.so, the MI355X (gfx950) build of
+ * PeakPatch.jl's halo-finding hot path: filter-bank multiply, local-maximum
+ * peak stencil, and per-peak radial-shell analysis with the ellipsoidal
+ * collapse-table lookup.
+ *
+ * Conventions
+ *  - Fields are Float32, n1 x n2 x n3, x fastest (Julia column-major layout,
+ *    zero-copy from a Julia Array{Float32,3}).
+ *  - Grid indices at this ABI are 0-based (the Julia shim subtracts 1).
+ *  - Every entry returns PP_OK (0) or a negative PP_ERR_* code; the message
+ *    of the last failure on the calling thread is pp_last_error().  Nothing
+ *    aborts or throws across the ABI.
+ *  - A pp_ctx is bound to one device and owns one HIP stream, the rocFFT plan
+ *    cache, the shell tables, the collapse table and pooled device buffers.
+ *    A context may be used by one host thread at a time.
+ *  - Host buffers are owned by the caller.  Pointers flagged on_device are
+ *    device memory of the context's device, also owned by the caller.
+ */
+#ifndef PEAKPATCH_HIP_H
+#define PEAKPATCH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_OK 0
+#define PP_ERR_ARG (-1)       /* bad argument / shape mismatch          */
+#define PP_ERR_OOM (-2)       /* device allocation failed              */
+#define PP_ERR_HIP (-3)       /* HIP runtime error                     */
+#define PP_ERR_FFT (-4)       /* rocFFT error                          */
+#define PP_ERR_STATE (-5)     /* missing table / wrong call order      */
+#define PP_ERR_CAPACITY (-6)  /* caller's output arrays too small      */
+#define PP_ERR_BUSY (-7)      /* serving queue full / not admitted     */
+#define PP_ERR_TIMEOUT (-8)   /* serving wait timed out                */
+#define PP_ERR_INJECTED (-9)  /* fault injected for failure tests      */
+
+/* smooth_field window selector (Filters.jl:143-186, wsmooth) */
+#define PP_WINDOW_GAUSSIAN 0  /* exp(-k^2 R^2 / 8)          */
+#define PP_WINDOW_TOPHAT 1    /* 3 (sin x - x cos x) / x^3  */
+#define PP_WINDOW_K2GAUSS 3   /* k^2 exp(-k^2 R^2 / 8)      */
+
+/* shell-analysis kernel form */
+#define PP_SHELL_PLAIN 0      /* one workgroup per peak, one launch per batch */
+#define PP_SHELL_PERSISTENT 1 /* persistent work-queue kernel, radius-sorted */
+
+/* analyse_peak early-exit reasons (RadialShell.jl:14-34 dump counters) */
+#define PP_DUMP_NONE 0
+#define PP_DUMP_PREMASK 1
+#define PP_DUMP_NO_FCRIT 2
+#define PP_DUMP_M0_ONE 3
+#define PP_DUMP_NO_COLLAPSE 4
+#define PP_DUMP_RTHL_NEG 5
+
+typedef struct pp_ctx pp_ctx;
+
+/* ---------------------------------------------------------------- basics */
+const char *pp_last_error(void);
+int pp_version(void);
+int pp_device_count(int *count);
+
+/* Replaces the CUDA.jl context set-up of ext/CUDAExt.jl (device selection,
+ * set_cuda_device! CUDAExt.jl:3322-3325; plan cache CUDAExt.jl:82-106). */
+int pp_ctx_create(int device, pp_ctx **out);
+int pp_ctx_destroy(pp_ctx *ctx);
+int pp_ctx_synchronize(pp_ctx *ctx);
+/* Free / total device memory as seen by this context (for admission). */
+int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
+/* ------------------------------------------------------- static tables */
+/* Collapse table upload: read_homeltab + CollapseTableInterp
+ * (src/EllipsoidalCollapse/CollapseTable.jl:102-141).  table is Nx*Ny*Nz
+ * Float32, x fastest; the bounds are the header values widened to Float64. */
+int pp_upload_collapse_table(pp_ctx *ctx, const float *table, int nx, int ny, int nz,
+                             double X1, double X2, double Y1, double Y2, double Z1, double Z2);
+
+/* precompute_shells(rmax) (src/HaloFinder/RadialShell.jl:160-175) grouped per
+ * distinct r^2 as build_shell_tables (src/ShellAnalysisGPU.jl:60-88). */
+int pp_build_shell_tables(pp_ctx *ctx, int rmax);
+int pp_shell_table_info(pp_ctx *ctx, int64_t *ncells, int32_t *nshells);
+
+/* ---------------------------------------------------------- filter bank */
+/* In-place or out-of-place k-space window multiply of smooth_field
+ * (src/HaloFinder/Filters.jl:159-184; ext/CUDAExt.jl:2959-3000): every
+ * element of delta_k (complex64, (n/2+1) x n x n, x fastest) is multiplied by
+ * Float32(W(|k| Rf)) computed in Float64; k = 0 is left unscaled; Nyquist kept.
+ * scale multiplies every element afterwards (1 for the bare window). */
+int pp_window_multiply(pp_ctx *ctx, const void *delta_k, void *out, int on_device, int n,
+                       double boxsize, double Rf, int wsmooth, float scale);
+
+/* smooth_field (Filters.jl:159-186): window multiply + normalised C2R.
+ * delta_k complex64 (n/2+1)xnxn; out Float32 n^3. */
+int pp_smooth_field(pp_ctx *ctx, const void *delta_k, float *out, int on_device, int n,
+                    double boxsize, double Rf, int wsmooth);
+
+/* rfft of a real n^3 Float32 field into complex64 (n/2+1)xnxn. */
+int pp_rfft(pp_ctx *ctx, const float *in, void *out, int on_device, int n);
+
+typedef struct {
+    int64_t capacity;   /* in: length of each array below                    */
+    int64_t count;      /* out: peaks written                                */
+    int64_t needed;     /* out: peaks found (> capacity => PP_ERR_CAPACITY)  */
+    int32_t *i, *j, *k; /* 0-based grid indices                              */
+    int32_t *filt;      /* index into the Rf[] array that found the peak     */
+    float *delta_s;     /* smoothed delta at the peak (FcRf)                 */
+    float *lapd_s;      /* k^2-Gaussian smoothed field at the peak (d2Rf);
+                           may be NULL                                       */
+} pp_peaks;
+
+/* find_peaks on one filtered field (src/HaloFinder/PeakFind.jl:15-78): strict
+ * 27-point local maxima with ff >= fcrit over the inner cube, masked cells
+ * skipped, each found peak cell masked.  Deterministic order k, j, i as the
+ * CPU loop; at most max_peaks (the CPU early return, PeakFind.jl:70-72).
+ * delta_s and mask are device or host as on_device says (mask n^3 Int8). */
+int pp_find_peaks_field(pp_ctx *ctx, const float *delta_s, int8_t *mask, int on_device,
+                        int n1, int n2, int n3, int nbuff, float fcrit, int64_t max_peaks,
+                        pp_peaks *out);
+
+/* peak_find_tile_gpu (ext/CUDAExt.jl:3068-3197; src/PeakPatch.jl:341-366) and
+ * the CPU filter loop (src/Pipeline.jl:160-195; MultiResolution.jl:885-907):
+ * rfft(delta) once, then per filter in the given order (the driver passes
+ * descending Rf): window -> C2R -> stencil (mask shared across filters), plus
+ * the wsmooth=3 field sampled at the new peaks when want_lapd.  Output order:
+ * filter order, then k, j, i.  mask_inout (n^3 Int8, may be NULL = zeros)
+ * is read and updated.  max_peaks_per_filter <= 0 means 2,000,000. */
+int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double boxsize,
+                  const double *Rf, const float *fcrit, int nfilt, int wsmooth, int want_lapd,
+                  int nbuff, int64_t max_peaks_per_filter, int8_t *mask_inout, pp_peaks *out);
+
+/* --------------------------------------------------------- shell analysis */
+typedef struct {
+    int n1, n2, n3;
+    const float *delta, *etax, *etay, *etaz; /* required                     */
+    const float *eta2x, *eta2y, *eta2z;      /* NULL when ilpt < 2          */
+    const float *lapd;                       /* NULL when ioutshear == 0    */
+    int on_device;                           /* 1: device pointers           */
+} pp_fields;
+
+/* One batch of peaks: the tuple (pi, pj, pk, Rf, ir2min, ZZon_pp, fcrit_pp)
+ * of analyse_peaks_gpu_cuda_multirf (ext/CUDAExt.jl:2246-2258, 2326-2334), one
+ * row per peak.  Host arrays. */
+typedef struct {
+    int64_t npeaks;
+    const int32_t *i, *j, *k; /* 0-based                                   */
+    const double *Rf;         /* Rfclvi                                    */
+    const int64_t *ir2min;
+    const double *ZZon;       /* 1 + z of the peak                         */
+    const double *fcrit;      /* collapse threshold (fsc_of_z or override) */
+} pp_peak_batch;
+
+/* PeakResult (RadialShell.jl:58-72) as SoA host arrays, Float64.  Vectors
+ * are 3 per peak, strain 9 per peak (column-major 3x3).  Any pointer may be
+ * NULL to skip that output. */
+typedef struct {
+    double *RTHL, *Srb, *Sbar, *Sbar2, *strain, *gradpk, *gradpkrf;
+    double *Fbarx, *e_v, *p_v, *zvir_half, *gradpkf, *d2F;
+    int32_t *status;          /* PP_DUMP_* per peak                        */
+    int32_t *nshells;         /* finalised shells per peak (diagnostic)    */
+    int64_t dump_counts[6];   /* out: per PP_DUMP_* totals (get_dump_counts) */
+} pp_results;
+
+/* analyse_peak over a batch (src/HaloFinder/RadialShell.jl:282-891 semantics;
+ * replaces analyse_peaks_gpu_cuda_multirf, ext/CUDAExt.jl:2246-2462, and the
+ * threaded CPU loop src/Pipeline.jl:233-241).  rmax2rs > 0 limits each peak
+ * to npart = floor(4/3 pi (rmax2rs Rf / alatt)^3) cells.  mask (n^3 Int8,
+ * host or device per mask_on_device, may be NULL) receives 1 for the cells
+ * within R_TH inside [nbuff, n - nbuff) (RadialShell.jl:792-802).  Requires
+ * pp_build_shell_tables and pp_upload_collapse_table first. */
+int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks,
+                     double alatt, double rmax2rs, int nbuff, int8_t *mask, int mask_on_device,
+                     int kernel_form, pp_results *out);
+
+/* Per-peak radial profiles of the same gather (shell counts and cumulative
+ * mean overdensity Fbar, RadialShell.jl:379-453), for profile parity tests:
+ * nshell_out and Fbar_out are npeaks x max_shells (row per peak); m_out the
+ * finalised shell count per peak. */
+int pp_shell_profiles(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks,
+                      double alatt, double rmax2rs, int kernel_form, int max_shells,
+                      int32_t *m_out, int32_t *nshell_out, double *Fbar_out);
+
+/* Autotuner: times the persistent kernel on the given batch for each
+ * candidate launch shape and keeps the fastest per radius class in ctx.
+ * Returns the chosen workgroups-per-CU in *choice (may be NULL). */
+int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks,
+                      double alatt, double rmax2rs, int *choice);
+
+/* Kernel timing of the last pp_analyse_peaks / pp_find_peaks call (HIP
+ * events on the context stream), milliseconds. */
+int pp_last_timing(pp_ctx *ctx, double *shell_gather_ms, double *shell_total_ms,
+                   double *peakfind_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PEAKPATCH_HIP_H */

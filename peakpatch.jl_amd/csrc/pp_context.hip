@@ -1,0 +1,269 @@
+// pp_context.hip -- context, error reporting, static tables (collapse table,
+// shell tables) and the rocFFT plan cache of libpeakpatch_hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "pp_common.h"
+
+namespace pp {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int DevBuf::grow(size_t need)
+{
+    if (need <= bytes && ptr) return PP_OK;
+    release();
+    size_t b = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        set_error("hipMalloc(%zu bytes) failed: %s", b, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? PP_ERR_OOM : PP_ERR_HIP;
+    }
+    bytes = b;
+    return PP_OK;
+}
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int ensure_device(pp_ctx *ctx)
+{
+    PP_HIP(hipSetDevice(ctx->device));
+    return PP_OK;
+}
+
+static std::once_flag g_fft_once;
+
+int fft_prepare(pp_ctx *ctx, int n)
+{
+    FftPlans &F = ctx->fft;
+    if (F.n == n && F.r2c) return PP_OK;
+    std::call_once(g_fft_once, [] { rocfft_setup(); });
+    if (F.r2c) rocfft_plan_destroy(F.r2c);
+    if (F.c2r) rocfft_plan_destroy(F.c2r);
+    if (F.info) rocfft_execution_info_destroy(F.info);
+    F.r2c = F.c2r = nullptr;
+    F.info = nullptr;
+    F.n = 0;
+    // lengths are fastest-dimension first; the field is x fastest.
+    size_t len[3] = {(size_t)n, (size_t)n, (size_t)n};
+    PP_FFT(rocfft_plan_create(&F.r2c, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                              rocfft_precision_single, 3, len, 1, nullptr));
+    PP_FFT(rocfft_plan_create(&F.c2r, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                              rocfft_precision_single, 3, len, 1, nullptr));
+    size_t w1 = 0, w2 = 0;
+    PP_FFT(rocfft_plan_get_work_buffer_size(F.r2c, &w1));
+    PP_FFT(rocfft_plan_get_work_buffer_size(F.c2r, &w2));
+    size_t w = std::max(w1, w2);
+    PP_FFT(rocfft_execution_info_create(&F.info));
+    PP_FFT(rocfft_execution_info_set_stream(F.info, ctx->stream));
+    if (w > 0) {
+        PP_CHECK(F.work.grow(w));
+        PP_FFT(rocfft_execution_info_set_work_buffer(F.info, F.work.ptr, F.work.bytes));
+    }
+    F.n = n;
+    return PP_OK;
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+extern "C" {
+
+const char *pp_last_error(void) { return g_err; }
+
+int pp_version(void) { return 10000; }
+
+int pp_device_count(int *count)
+{
+    PP_ARG(count != nullptr, "count is NULL");
+    int c = 0;
+    PP_HIP(hipGetDeviceCount(&c));
+    *count = c;
+    return PP_OK;
+}
+
+int pp_ctx_create(int device, pp_ctx **out)
+{
+    PP_ARG(out != nullptr, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    PP_HIP(hipGetDeviceCount(&ndev));
+    PP_ARG(device >= 0 && device < ndev, "device %d out of range (%d devices)", device, ndev);
+    pp_ctx *c = new pp_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) {
+        set_error("context creation on device %d failed: %s", device, hipGetErrorString(e));
+        delete c;
+        return PP_ERR_HIP;
+    }
+    c->num_cu = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    *out = c;
+    return PP_OK;
+}
+
+int pp_ctx_destroy(pp_ctx *ctx)
+{
+    if (!ctx) return PP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    FftPlans &F = ctx->fft;
+    if (F.r2c) rocfft_plan_destroy(F.r2c);
+    if (F.c2r) rocfft_plan_destroy(F.c2r);
+    if (F.info) rocfft_execution_info_destroy(F.info);
+    F.work.release();
+    DevBuf *bufs[] = {&ctx->d_table, &ctx->d_field_a, &ctx->d_field_b, &ctx->d_k0, &ctx->d_k1,
+                      &ctx->d_k2, &ctx->d_mask, &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan,
+                      &ctx->d_cub, &ctx->d_pk, &ctx->d_res, &ctx->d_prof, &ctx->d_queue,
+                      &ctx->shells.d_cell, &ctx->shells.d_shell_start, &ctx->shells.d_shell_r2,
+                      &ctx->shells.d_shell_of_r2, &ctx->shells.d_groups};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &b : ctx->d_in) b.release();
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PP_OK;
+}
+
+int pp_ctx_synchronize(pp_ctx *ctx)
+{
+    PP_ARG(ctx, "ctx is NULL");
+    PP_CHECK(ensure_device(ctx));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    return PP_OK;
+}
+
+int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes)
+{
+    PP_ARG(ctx && free_bytes && total_bytes, "NULL argument");
+    PP_CHECK(ensure_device(ctx));
+    PP_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return PP_OK;
+}
+
+int pp_upload_collapse_table(pp_ctx *ctx, const float *table, int nx, int ny, int nz, double X1,
+                             double X2, double Y1, double Y2, double Z1, double Z2)
+{
+    PP_ARG(ctx && table, "NULL argument");
+    PP_ARG(nx >= 2 && ny >= 2 && nz >= 2, "collapse table needs >= 2 points per axis (%d,%d,%d)",
+           nx, ny, nz);
+    PP_ARG(X2 > X1 && Y2 > Y1 && Z2 > Z1, "collapse table bounds must be increasing");
+    PP_CHECK(ensure_device(ctx));
+    size_t bytes = (size_t)nx * ny * nz * sizeof(float);
+    PP_CHECK(ctx->d_table.grow(bytes));
+    PP_HIP(hipMemcpyAsync(ctx->d_table.ptr, table, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->tnx = nx; ctx->tny = ny; ctx->tnz = nz;
+    ctx->tX1 = X1; ctx->tX2 = X2; ctx->tY1 = Y1; ctx->tY2 = Y2; ctx->tZ1 = Z1; ctx->tZ2 = Z2;
+    ctx->have_table = true;
+    return PP_OK;
+}
+
+
+
+int pp_build_shell_tables(pp_ctx *ctx, int rmax)
+{
+    PP_ARG(ctx, "ctx is NULL");
+    PP_ARG(rmax >= 1 && rmax <= 120, "rmax %d outside [1, 120]", rmax);
+    PP_CHECK(ensure_device(ctx));
+    ShellTable &T = ctx->shells;
+    if (T.rmax == rmax) return PP_OK;
+    const int r2max = rmax * rmax;
+    // counting sort on r^2 keeps generation order (dk outer, dj, di inner)
+    // inside each r^2: a stable sort as RadialShell.jl:160-175.
+    std::vector<int64_t> count((size_t)r2max + 2, 0);
+    for (int k = -rmax; k <= rmax; k++)
+        for (int j = -rmax; j <= rmax; j++)
+            for (int i = -rmax; i <= rmax; i++) {
+                int q = i * i + j * j + k * k;
+                if (q <= r2max) count[(size_t)q + 1]++;
+            }
+    for (int q = 1; q <= r2max + 1; q++) count[q] += count[q - 1];
+    int64_t ncells = count[(size_t)r2max + 1];
+    T.di.assign(ncells, 0); T.dj.assign(ncells, 0); T.dk.assign(ncells, 0); T.r2.assign(ncells, 0);
+    std::vector<int64_t> pos(count.begin(), count.end() - 1);
+    for (int k = -rmax; k <= rmax; k++)
+        for (int j = -rmax; j <= rmax; j++)
+            for (int i = -rmax; i <= rmax; i++) {
+                int q = i * i + j * j + k * k;
+                if (q > r2max) continue;
+                int64_t p = pos[q]++;
+                T.di[p] = i; T.dj[p] = j; T.dk[p] = k; T.r2[p] = q;
+            }
+    T.shell_start.clear();
+    T.shell_r2.clear();
+    std::vector<int32_t> shell_of_r2((size_t)r2max + 1, 0);
+    for (int64_t c = 0; c < ncells; c++) {
+        if (c == 0 || T.r2[c] != T.r2[c - 1]) {
+            T.shell_start.push_back((int32_t)c);
+            T.shell_r2.push_back(T.r2[c]);
+            shell_of_r2[T.r2[c]] = (int32_t)T.shell_r2.size();  // 1-based
+        }
+    }
+    T.shell_start.push_back((int32_t)ncells);
+    T.nshells = (int32_t)T.shell_r2.size();
+    T.ncells = ncells;
+    std::vector<int32_t> packed(ncells);
+    for (int64_t c = 0; c < ncells; c++)
+        packed[c] = (T.di[c] + 128) | ((T.dj[c] + 128) << 8) | ((T.dk[c] + 128) << 16);
+    PP_CHECK(T.d_cell.grow(ncells * sizeof(int32_t)));
+    PP_CHECK(T.d_shell_start.grow(T.shell_start.size() * sizeof(int32_t)));
+    PP_CHECK(T.d_shell_r2.grow(T.shell_r2.size() * sizeof(int32_t)));
+    PP_CHECK(T.d_shell_of_r2.grow(shell_of_r2.size() * sizeof(int32_t)));
+    PP_HIP(hipMemcpy(T.d_cell.ptr, packed.data(), ncells * sizeof(int32_t), hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(T.d_shell_start.ptr, T.shell_start.data(), T.shell_start.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(T.d_shell_r2.ptr, T.shell_r2.data(), T.shell_r2.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(T.d_shell_of_r2.ptr, shell_of_r2.data(), shell_of_r2.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+    T.rmax = rmax;
+    PP_CHECK(pp::shell_groups_build(ctx));
+    return PP_OK;
+}
+
+int pp_shell_table_info(pp_ctx *ctx, int64_t *ncells, int32_t *nshells)
+{
+    PP_ARG(ctx && ncells && nshells, "NULL argument");
+    if (ctx->shells.rmax < 0) {
+        set_error("shell tables not built (call pp_build_shell_tables)");
+        return PP_ERR_STATE;
+    }
+    *ncells = ctx->shells.ncells;
+    *nshells = ctx->shells.nshells;
+    return PP_OK;
+}
+
+int pp_last_timing(pp_ctx *ctx, double *g, double *s, double *p)
+{
+    PP_ARG(ctx, "ctx is NULL");
+    if (g) *g = ctx->timing.gather_ms;
+    if (s) *s = ctx->timing.shell_ms;
+    if (p) *p = ctx->timing.peakfind_ms;
+    return PP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,549 @@
+// pp_post.h -- device-side per-peak analysis after the shell gather: the
+// Fbar recursion, fcrit crossing, inward virialisation walk, R_TH
+// interpolation and the final catalogue quantities of analyse_peak
+// (src/HaloFinder/RadialShell.jl:379-891).
+//
+// The whole wave executes one peak's analysis with wave-uniform control
+// flow; the radial sums (kernel_strain, Sbar, Srb, d2F) are split across the
+// 64 lanes and combined with a fixed butterfly, so results are deterministic.
+// Everything here is Float64, including the strain eigenvalues and the
+// collapse-table interpolation that decide zvir >= ZZon.
+//
+// Where the reference scans a monotone radius table linearly (mlow / mupp /
+// m0_rc searches) a binary search is used; rad[] is strictly increasing so
+// the index found is the same.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pp {
+
+// Profile field order (per shell): count, F, S(3), S2(3), G(3), Gf(3), SR(9, L*3+K), lapd
+enum : int { PF_N = 0, PF_F = 1, PF_S = 2, PF_S2 = 5, PF_G = 8, PF_GF = 11, PF_SR = 14, PF_L = 23, PF_NFIELD = 24 };
+
+// Result record (doubles per peak)
+enum : int {
+    R_RTHL = 0, R_SRB = 1, R_SBAR = 2, R_SBAR2 = 5, R_STRAIN = 8, R_GRADPK = 17, R_GRADPKRF = 20,
+    R_FBARX = 23, R_EV = 24, R_PV = 25, R_ZVIRH = 26, R_GRADPKF = 27, R_D2F = 30, R_NREC = 32
+};
+
+enum : int { DUMP_NONE = 0, DUMP_PREMASK = 1, DUMP_NO_FCRIT = 2, DUMP_M0_ONE = 3, DUMP_NO_COLLAPSE = 4,
+             DUMP_RTHL_NEG = 5, DUMP_NONFINITE = 6 };
+
+struct PeakParam {
+    int32_t ci, cj, ck;  // 0-based centre
+    int32_t m;           // finalised shells (1-based count, centre included)
+    int32_t m0;          // initial m0 from the gather (first shell with r2 > ir2min)
+    int32_t mrf;         // Phase-3 shell (with the reference's one-shell lag)
+    int32_t npart;       // cells considered (rmax2rs cut)
+    int32_t flags;       // bit0: gather sphere fully inside the grid; bit1: premask dump
+    double ZZon, fcrit;
+    int64_t order;       // caller's index of this peak
+};
+
+struct Table3 {
+    const float *data;
+    int nx, ny, nz;
+    double X1, X2, Y1, Y2, Z1, Z2;
+};
+
+struct PostConst {
+    const double *rad;    // rad[s-1] = sqrt(r2 of shell s)
+    const double *akk;    // atab4, 1101 entries
+    const int32_t *cell;  // packed offsets, r^2 sorted
+    const int32_t *shell_start;
+    Table3 tab;
+    double wRnor, aRnor;
+    int n1, n2, n3, nbuff;
+    const float *lapd;    // for the rare out-of-bounds d2F walk
+    int8_t *mask;         // NULL: no mask output
+};
+
+// ----------------------------------------------------------- wave helpers
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sumf(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// hRinteg (RadialShell.jl:100-127)
+__device__ __forceinline__ double d_hRinteg(double u0, double u12)
+{
+    double x0 = fabs(u0);
+    if (x0 >= 2.0) return 0.0;
+    const double one7 = 1.0 / 7.0;
+    double result = 1.6 * u12 - 6.4 * one7;
+    double x02 = x0 * x0, x04 = x02 * x02;
+    if (x0 >= 1.0) {
+        double h1 = x04 * (2.0 - x0 * (2.4 - x0 * (1.0 - x0 * one7)));
+        double h2 = x02 * (4.0 - x0 * (4.0 - x0 * (1.5 - x0 * 0.2)));
+        result = result - h2 * u12 + h1;
+    } else {
+        result = result - 0.2 * (u12 - one7);
+        if (x0 > 0.0) {
+            double h1 = x04 * (1.0 - x02 * (1.0 - 3.0 * x0 * one7));
+            double h2 = x02 * (2.0 - x02 * (1.5 - 0.6 * x0));
+            result = result - h2 * u12 + h1;
+        }
+    }
+    return result;
+}
+
+// Symmetric 3x3 eigenvalues, ascending, closed form (trigonometric solution
+// of the characteristic cubic; Smith 1961).  a = {a00, a11, a22, a01, a02, a12}.
+__device__ __forceinline__ void d_eig3(const double a[6], double lam[3])
+{
+    double p1 = a[3] * a[3] + a[4] * a[4] + a[5] * a[5];
+    if (p1 == 0.0) {
+        double x = a[0], y = a[1], z = a[2], t;
+        if (x > y) { t = x; x = y; y = t; }
+        if (y > z) { t = y; y = z; z = t; }
+        if (x > y) { t = x; x = y; y = t; }
+        lam[0] = x; lam[1] = y; lam[2] = z;
+        return;
+    }
+    double q = (a[0] + a[1] + a[2]) / 3.0;
+    double b0 = a[0] - q, b1 = a[1] - q, b2 = a[2] - q;
+    double p2 = b0 * b0 + b1 * b1 + b2 * b2 + 2.0 * p1;
+    double p = sqrt(p2 / 6.0);
+    double det = b0 * (b1 * b2 - a[5] * a[5]) - a[3] * (a[3] * b2 - a[5] * a[4]) + a[4] * (a[3] * a[5] - b1 * a[4]);
+    double r = det / (2.0 * p * p * p);
+    r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+    double phi = acos(r) / 3.0;
+    double l3 = q + 2.0 * p * cos(phi);
+    double l1 = q + 2.0 * p * cos(phi + 2.0943951023931954923);
+    double l2 = 3.0 * q - l1 - l3;
+    lam[0] = l1; lam[1] = l2; lam[2] = l3;
+}
+
+// interpolate(ct, x, y, z) (CollapseTable.jl:128-145): trilinear on the
+// evenly spaced table, -1 outside [X1,X2]x[Y1,Y2]x[Z1,Z2].
+__device__ __forceinline__ void d_lin_pos(double x, double lo, double hi, int n, int &i0, double &f)
+{
+    double step = (hi - lo) / (double)(n - 1);
+    double p = (x - lo) / step + 1.0;
+    double fl = floor(p);
+    if (p == (double)n) fl -= 1.0;
+    if (fl < 1.0) fl = 1.0;
+    if (fl > (double)(n - 1)) fl = (double)(n - 1);
+    i0 = (int)fl - 1;
+    f = p - fl;
+}
+
+__device__ __forceinline__ double d_interp(const Table3 &t, double x, double y, double z)
+{
+    if (!(x >= t.X1 && x <= t.X2 && y >= t.Y1 && y <= t.Y2 && z >= t.Z1 && z <= t.Z2)) return -1.0;
+    int ix, iy, iz;
+    double fx, fy, fz;
+    d_lin_pos(x, t.X1, t.X2, t.nx, ix, fx);
+    d_lin_pos(y, t.Y1, t.Y2, t.ny, iy, fy);
+    d_lin_pos(z, t.Z1, t.Z2, t.nz, iz, fz);
+    const long nx = t.nx, nxy = (long)t.nx * t.ny;
+    const float *d = t.data + ix + iy * nx + iz * nxy;
+    double c00 = (double)d[0] * (1.0 - fx) + (double)d[1] * fx;
+    double c10 = (double)d[nx] * (1.0 - fx) + (double)d[nx + 1] * fx;
+    double c01 = (double)d[nxy] * (1.0 - fx) + (double)d[nxy + 1] * fx;
+    double c11 = (double)d[nxy + nx] * (1.0 - fx) + (double)d[nxy + nx + 1] * fx;
+    double c0 = c00 * (1.0 - fy) + c10 * fy;
+    double c1 = c01 * (1.0 - fy) + c11 * fy;
+    return c0 * (1.0 - fz) + c1 * fz;
+}
+
+// first s in [lo, hi] (1-based) with rad(s) > x, or hi+1
+template <class PV>
+__device__ __forceinline__ int first_rad_gt(const PV &P, int lo, int hi, double x)
+{
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (P.rad(mid) > x) hi = mid - 1; else lo = mid + 1;
+    }
+    return lo;
+}
+// last s in [lo, hi] with rad(s) < x, or lo-1
+template <class PV>
+__device__ __forceinline__ int last_rad_lt(const PV &P, int lo, int hi, double x)
+{
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (P.rad(mid) < x) lo = mid + 1; else hi = mid - 1;
+    }
+    return hi;
+}
+
+struct Strain {
+    double E[6];  // Ebar a00, a11, a22, a01, a02, a12
+    double g[3], gf[3];
+};
+
+// kernel_strain (RadialShell.jl:228-277), wave-parallel over m1.
+template <class PV>
+__device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlow, int mupp)
+{
+    const int lane = threadIdx.x & 63;
+    double E[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
+    const double rmp = P.rad(mp);
+    for (int m1 = mlow + lane; m1 <= mupp; m1 += 64) {
+        double wt;
+        const double r1 = P.rad(m1);
+        if (mp > 1) {
+            double u0 = rmp - r1;
+            double u02 = u0 * u0;
+            if (!(u02 < 4.0)) continue;
+            double u12 = rmp * rmp + r1 * r1;
+            double den = u12 - u02;
+            wt = d_hRinteg(u0, u12) / (den * den);
+        } else {
+            double u0 = r1;
+            double con = 100.0 * u0 * u0;
+            int icon = (int)floor(con);
+            double diff = con - (double)icon;
+            double aww = C.akk[icon] + diff * (C.akk[icon + 1] - C.akk[icon]);
+            wt = 0.5 * aww / r1;
+        }
+        double G[3], Gf[3], SR[9];
+        P.strain_inputs(m1, G, Gf, SR);
+#pragma unroll
+        for (int L = 0; L < 3; L++) {
+            g[L] += wt * G[L];
+            gf[L] += wt * Gf[L];
+        }
+        E[0] -= 0.5 * wt * (SR[0] + SR[0]);
+        E[1] -= 0.5 * wt * (SR[4] + SR[4]);
+        E[2] -= 0.5 * wt * (SR[8] + SR[8]);
+        E[3] -= 0.5 * wt * (SR[1] + SR[3]);
+        E[4] -= 0.5 * wt * (SR[2] + SR[6]);
+        E[5] -= 0.5 * wt * (SR[5] + SR[7]);
+    }
+    Strain s;
+    const double scale = mp > 1 ? C.wRnor / rmp : C.aRnor;
+#pragma unroll
+    for (int i = 0; i < 6; i++) s.E[i] = wave_sum(E[i]) * scale;
+#pragma unroll
+    for (int L = 0; L < 3; L++) {
+        s.g[L] = wave_sum(g[L]) * scale;
+        s.gf[L] = wave_sum(gf[L]) * scale;
+    }
+    return s;
+}
+
+__device__ __forceinline__ bool finite6(const double E[6])
+{
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 6; i++) ok = ok && isfinite(E[i]);
+    return ok;
+}
+
+__device__ __forceinline__ void evals_ep(const double E[6], double &Frho, double &e_v, double &p_v)
+{
+    double lam[3];
+    d_eig3(E, lam);
+    Frho = lam[0] + lam[1] + lam[2];
+    e_v = 0.0;
+    p_v = 0.0;
+    if (Frho > 0.0) {
+        e_v = 0.5 * (lam[2] - lam[0]) / Frho;
+        p_v = 0.5 * (lam[2] + lam[0] - 2.0 * lam[1]) / Frho;
+    }
+}
+
+// The per-peak analysis.  P provides rad(s), Fbar(s), count(s), sums of S /
+// S2 / lapd and strain_inputs(s) (G, Gf, SR already divided by rad, zero at
+// s = 1).  res: R_NREC doubles.  Returns the dump reason.
+template <class PV>
+__device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C, double *res)
+{
+    const int lane = threadIdx.x & 63;
+    const int m = pk.m;
+    int status = DUMP_NONE;
+    double RTHL = -1.0;
+    if (pk.flags & 2) { status = DUMP_PREMASK; goto nocol; }
+    {
+        // ---- Phase 3: gradient at Rf (RadialShell.jl:456-490) ----
+        const int mrf = pk.mrf;
+        double rlow_rf = fmax(P.rad(mrf) - 2.0, 0.0);
+        int mlow_rf = mrf;
+        if (mrf > 1) { int a = first_rad_gt(P, 1, mrf - 1, rlow_rf); if (a <= mrf - 1) mlow_rf = a; }
+        int mupp_rf = mrf;
+        { int b = last_rad_lt(P, mrf + 1, m, P.rad(mrf) + 2.0); if (b >= mrf + 1) mupp_rf = b; }
+        Strain srf = kernel_strain(P, C, mrf, mlow_rf, mupp_rf);
+        const double gradpkrf[3] = {srf.g[0], srf.g[1], srf.g[2]};
+
+        // ---- Phase 4: fcrit crossing (RadialShell.jl:492-538) ----
+        const double fcrit = pk.fcrit;
+        int m0 = pk.m0, mupp;
+        if (P.Fbar(m0) >= fcrit) {
+            int found = 0;
+            for (int base = m0; base <= m; base += 64) {
+                int mm = base + lane;
+                bool hit = mm <= m && P.Fbar(mm) < fcrit;
+                unsigned long long b = __ballot(hit);
+                if (b) { m0 = base + __ffsll((long long)b) - 1; found = 1; break; }
+            }
+            if (!found) mupp = m;
+            else {
+                int b = last_rad_lt(P, m0 + 1, m, P.rad(m0) + 2.0);
+                mupp = b >= m0 + 1 ? b : m0;
+            }
+        } else {
+            int mstart = m0 - 1 > 1 ? m0 - 1 : 1;
+            int found = 0;
+            for (int top = mstart; top >= 1; top -= 64) {
+                int mp = top - lane;
+                bool hit = mp >= 1 && P.Fbar(mp) >= fcrit;
+                unsigned long long b = __ballot(hit);
+                if (b) { m0 = top - (__ffsll((long long)b) - 1) + 1; found = 1; break; }
+            }
+            if (!found) { status = DUMP_NO_FCRIT; goto nocol; }
+            int b = last_rad_lt(P, m0 + 1, m, P.rad(m0) + 2.0);
+            mupp = b >= m0 + 1 ? b : m0;
+        }
+
+        // ---- Phase 5: inward walk (RadialShell.jl:540-704) ----
+        double rlow = fmax(P.rad(m0) - 2.0, 0.0);
+        int mlow = m0;
+        if (m0 > 1) { int a = first_rad_gt(P, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
+        const double ZZon = pk.ZZon;
+        Strain s0 = kernel_strain(P, C, m0, mlow, mupp);
+        if (!finite6(s0.E)) { status = DUMP_NONFINITE; goto nocol; }
+        double Frho_m0, e_v_m0, p_v_m0;
+        evals_ep(s0.E, Frho_m0, e_v_m0, p_v_m0);
+        double zvir1 = -1.0, zvir1p = -1.0;
+        Strain last = s0;               // Ebar_last, gradpk_last, gradpkf_last
+        double strain_mat[6];           // the previous step's Ebar
+        double gradpk_prev[3], gradpkf_prev[3];
+        double Frhoh = Frho_m0, Frho = P.Fbar(m0), e_v = e_v_m0, p_v = p_v_m0;
+        for (int i = 0; i < 6; i++) strain_mat[i] = s0.E[i];
+        for (int L = 0; L < 3; L++) { gradpk_prev[L] = s0.g[L]; gradpkf_prev[L] = s0.gf[L]; }
+        // zvir1p at m0 is -1 (RadialShell.jl:581-583), never >= ZZon > 0 branch
+        // is kept for ZZon <= -1 inputs:
+        bool collapsed = false;
+        if (-1.0 >= ZZon) {
+            zvir1 = -1.0;
+            zvir1p = -1.0;
+            collapsed = true;
+        } else {
+            zvir1 = -1.0;
+            if (m0 == 1) { status = DUMP_M0_ONE; goto nocol; }
+            for (int mp = m0 - 1; mp >= 1; mp--) {
+                double rupp_mp = P.rad(mp) + 2.0;
+                double rlow_mp = fmax(P.rad(mp) - 2.0, 0.0);
+                if (P.rad(mupp) > rupp_mp) {
+                    int b = last_rad_lt(P, mp + 1, mupp, rupp_mp);
+                    mupp = b >= mp + 1 ? b : mp;
+                }
+                if (mlow > 1 && P.rad(mlow - 1) > rlow_mp) {
+                    int a = first_rad_gt(P, 1, mlow - 1, rlow_mp);
+                    mlow = a <= mlow - 1 ? a : mlow;
+                }
+                Strain smp = kernel_strain(P, C, mp, mlow, mupp);
+                if (!finite6(smp.E)) { status = DUMP_NONFINITE; goto nocol; }
+                double Frho_mp, e_v_mp, p_v_mp;
+                evals_ep(smp.E, Frho_mp, e_v_mp, p_v_mp);
+                double Frhoh_mp = Frho_mp;
+                Frho_mp = P.Fbar(mp);
+                double zvir1p_mp = -1.0;
+                if (Frho_mp > 0.0) {
+                    double poe = e_v_mp < 1e-5 ? 0.0 : p_v_mp / e_v_mp;
+                    zvir1p_mp = d_interp(C.tab, log10(Frho_mp), e_v_mp, poe);
+                }
+                if (zvir1p_mp >= ZZon) {
+                    m0 = mp + 1;
+                    zvir1p = zvir1p_mp;
+                    last = smp;
+                    Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
+                    collapsed = true;
+                    break;
+                }
+                zvir1 = zvir1p_mp;
+                for (int L = 0; L < 3; L++) { gradpk_prev[L] = smp.g[L]; gradpkf_prev[L] = smp.gf[L]; }
+                for (int i = 0; i < 6; i++) strain_mat[i] = smp.E[i];
+                Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
+                last = smp;
+            }
+        }
+        if (!collapsed) { status = DUMP_NO_COLLAPSE; goto nocol; }
+        // only reachable with ZZon <= -1 (collapse flagged at m0 itself): the
+        // reference would index rad[0]; report it as the m0 == 1 dump.
+        if (m0 < 2) { status = DUMP_M0_ONE; goto nocol; }
+
+        // ---- Phase 6 (RadialShell.jl:706-754) ----
+        const double r0 = P.rad(m0 - 1), r1 = P.rad(m0);
+        const double rad3p = r0 * r0 * r0, rad3 = r1 * r1 * r1;
+        double dZvir = zvir1p - zvir1;
+        if (zvir1 > 0.0 && dZvir != 0.0) {
+            double RTHL3 = rad3p + (rad3 - rad3p) * (zvir1p - ZZon) / dZvir;
+            RTHL = pow(RTHL3, 1.0 / 3.0);
+        } else {
+            RTHL = r0;
+        }
+        if (RTHL <= 0.0) { status = DUMP_RTHL_NEG; goto nocol; }
+        const double RTHL3 = RTHL * RTHL * RTHL, RTHL5 = RTHL3 * RTHL * RTHL;
+        const double drad3 = rad3 - rad3p;
+        const double dFbar = P.Fbar(m0) - P.Fbar(m0 - 1);
+        double Fbarx, smat[6], gpk[3], gpkf[3];
+        if (zvir1 > 0.0 && drad3 != 0.0) {
+            double frac = (RTHL3 - rad3p) / drad3;
+            Fbarx = P.Fbar(m0 - 1) + frac * dFbar;
+            for (int i = 0; i < 6; i++) smat[i] = last.E[i] + frac * (strain_mat[i] - last.E[i]);
+            for (int L = 0; L < 3; L++) {
+                gpk[L] = last.g[L] + frac * (gradpk_prev[L] - last.g[L]);
+                gpkf[L] = last.gf[L] + frac * (gradpkf_prev[L] - last.gf[L]);
+            }
+        } else {
+            Fbarx = P.Fbar(m0 - 1);
+            for (int i = 0; i < 6; i++) smat[i] = last.E[i];
+            for (int L = 0; L < 3; L++) { gpk[L] = gradpk_prev[L]; gpkf[L] = gradpkf_prev[L]; }
+        }
+        {   // normalize_strain (RadialShell.jl:209-212)
+            double tr = smat[0] + smat[1] + smat[2];
+            if (tr != 0.0) { double f = Fbarx / tr; for (int i = 0; i < 6; i++) smat[i] *= f; }
+        }
+        if (!finite6(smat)) { status = DUMP_NONFINITE; goto nocol; }
+        double Frhoc, e_v_f, p_v_f;
+        evals_ep(smat, Frhoc, e_v_f, p_v_f);
+        const double sfac = Fbarx / Frhoh;
+
+        // ---- Phase 7 (RadialShell.jl:756-819) ----
+        double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0, Srb = 0.0;
+        for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += 64) {
+            double cnt = P.count(m1);
+            if (cnt > 0.0) {
+                for (int L = 0; L < 3; L++) { Sb[L] += P.S(m1, L); Sb2[L] += P.S2(m1, L); }
+                nS += cnt;
+            }
+            if (m1 >= 2) {
+                double ra = P.rad(m1 - 1), rb = P.rad(m1);
+                double r5a = ra * ra * ra * ra * ra, r5b = rb * rb * rb * rb * rb;
+                if (m1 == 2) r5a = 0.0;
+                Srb += 0.5 * (P.Fbar(m1 - 1) + P.Fbar(m1)) * (r5b - r5a);
+            }
+        }
+        for (int L = 0; L < 3; L++) { Sb[L] = wave_sum(Sb[L]); Sb2[L] = wave_sum(Sb2[L]); }
+        nS = wave_sum(nS);
+        Srb = wave_sum(Srb);
+        if (nS > 0.0) for (int L = 0; L < 3; L++) { Sb[L] /= nS; Sb2[L] /= nS; }
+        if (zvir1 > 0.0 && dZvir != 0.0) {
+            double rp = m0 - 1 >= 2 ? P.rad(m0 - 1) : 0.0;
+            double rad5p = rp * rp * rp * rp * rp;
+            Srb += 0.5 * (P.Fbar(m0 - 1) + Fbarx) * (RTHL5 - rad5p);
+        }
+        if (Fbarx * RTHL5 > 0.0) Srb /= (Fbarx * RTHL5);
+
+        // cells with r <= RTHL: shells 1..q
+        const int q = last_rad_lt(P, 1, m, nextafter(RTHL, 1e300));
+        const int cend = min(C.shell_start[q], pk.npart);  // cells [0, cend)
+        const int Rq = (int)floor(P.rad(q));
+        const bool inside_q = pk.ci - Rq >= 0 && pk.ci + Rq < C.n1 && pk.cj - Rq >= 0 && pk.cj + Rq < C.n2 &&
+                              pk.ck - Rq >= 0 && pk.ck + Rq < C.n3;
+        if (C.mask) {
+            const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
+            const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
+            for (int c = lane; c < cend; c += 64) {
+                int pc = C.cell[c];
+                int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+                if (x < lo1 || x >= hi1 || y < lo2 || y >= hi2 || z < lo3 || z >= hi3) continue;
+                C.mask[x + (long)y * C.n1 + (long)z * C.n1 * C.n2] = 1;
+            }
+        }
+        double d2F = 0.0;
+        if (P.has_lapd()) {
+            if (inside_q) {
+                double sL = 0.0, sn = 0.0;
+                for (int m1 = 2 + lane; m1 <= q; m1 += 64) { sL += P.lap(m1); sn += P.count(m1); }
+                sL = wave_sum(sL);
+                sn = wave_sum(sn);
+                if (sn > 0.0) d2F = sL / sn;
+            } else if (lane == 0) {
+                // RadialShell.jl:804-819: stop at the first out-of-bounds cell
+                double s = 0.0;
+                long nav = 0;
+                for (int c = 1; c < cend; c++) {
+                    int pc = C.cell[c];
+                    int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+                    if (x < 0 || x >= C.n1 || y < 0 || y >= C.n2 || z < 0 || z >= C.n3) break;
+                    nav++;
+                    s += (double)C.lapd[x + (long)y * C.n1 + (long)z * C.n1 * C.n2];
+                }
+                d2F = nav > 0 ? s / (double)nav : 0.0;
+            }
+            d2F = __shfl(d2F, 0, 64);
+        }
+
+        // ---- zvir_half (RadialShell.jl:821-885) ----
+        double zvir_half = -1.0;
+        if (RTHL >= 3.0) {
+            for (int jj = 1; jj <= 10; jj++) {
+                double rcur = RTHL * pow((double)(jj - 1) / 9.0 * 0.5 + 0.5, 1.0 / 3.0);
+                double rupp_rc = rcur + 2.0;
+                // first m1 in 2..m-1 with rad > rupp_rc, else m-1 (1 if m <= 2)
+                int mupp_rc = 1;
+                if (m - 1 >= 2) {
+                    int f = first_rad_gt(P, 2, m - 1, rupp_rc);
+                    mupp_rc = f <= m - 1 ? f : m - 1;
+                }
+                double rlow_rc = fmax(rcur - 2.0, 0.0);
+                int mlow_rc = 1;
+                if (m - 1 >= 1) { int f = first_rad_gt(P, 1, m - 1, rlow_rc); if (f <= m - 1) mlow_rc = f; }
+                int m0_rc = 1;
+                if (m - 1 >= 1) {
+                    int f = first_rad_gt(P, 1, m - 1, rcur);
+                    m0_rc = f <= m - 1 ? f : m - 1;
+                }
+                Strain src = kernel_strain(P, C, m0_rc, mlow_rc, mupp_rc);
+                if (!finite6(src.E)) { status = DUMP_NONFINITE; goto nocol; }
+                double Frho_rc, e_rc, p_rc;
+                evals_ep(src.E, Frho_rc, e_rc, p_rc);
+                double Frhoc_rc = P.Fbar(m0_rc);
+                double poe_rc = e_rc < 1e-5 ? 0.0 : p_rc / e_rc;
+                double zvir_rc = -1.0;
+                if (Frhoc_rc > 0.0) zvir_rc = d_interp(C.tab, log10(Frhoc_rc), e_rc, poe_rc);
+                zvir_half = fmax(zvir_rc - 1.0, zvir_half);
+            }
+        }
+
+        if (lane == 0) {
+            res[R_RTHL] = RTHL;
+            res[R_SRB] = Srb;
+            for (int L = 0; L < 3; L++) {
+                res[R_SBAR + L] = Sb[L];
+                res[R_SBAR2 + L] = Sb2[L];
+                res[R_GRADPK + L] = gpk[L];
+                res[R_GRADPKRF + L] = gradpkrf[L];
+                res[R_GRADPKF + L] = gpkf[L];
+            }
+            // strain_mat = Ebar_last * (Fbarx / Frhoh), column-major 3x3
+            const double *E = last.E;
+            res[R_STRAIN + 0] = E[0] * sfac; res[R_STRAIN + 4] = E[1] * sfac; res[R_STRAIN + 8] = E[2] * sfac;
+            res[R_STRAIN + 1] = res[R_STRAIN + 3] = E[3] * sfac;
+            res[R_STRAIN + 2] = res[R_STRAIN + 6] = E[4] * sfac;
+            res[R_STRAIN + 5] = res[R_STRAIN + 7] = E[5] * sfac;
+            res[R_FBARX] = Fbarx;
+            res[R_EV] = e_v_f;
+            res[R_PV] = p_v_f;
+            res[R_ZVIRH] = zvir_half;
+            res[R_D2F] = d2F;
+            res[R_NREC - 1] = 0.0;
+        }
+        return DUMP_NONE;
+    }
+nocol:
+    if (lane == 0) {
+        for (int i = 0; i < R_NREC; i++) res[i] = 0.0;
+        res[R_RTHL] = -1.0;
+        res[R_ZVIRH] = -1.0;
+    }
+    return status;
+}
+
+}  // namespace pp

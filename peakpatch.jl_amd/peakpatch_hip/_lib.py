@@ -1,0 +1,102 @@
+"""ctypes binding of libpeakpatch_hip.so (include/peakpatch_hip.h).
+
+The shared library is built in-tree by ``peakpatch.jl_amd/build.py`` into
+``peakpatch.jl_amd/lib/libpeakpatch_hip.so``.  Loading fails loudly when it is
+missing: there is no CPU fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpeakpatch_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "peakpatch_hip.h")
+
+PP_OK = 0
+ERRORS = {-1: "PP_ERR_ARG", -2: "PP_ERR_OOM", -3: "PP_ERR_HIP", -4: "PP_ERR_FFT", -5: "PP_ERR_STATE",
+          -6: "PP_ERR_CAPACITY", -7: "PP_ERR_BUSY", -8: "PP_ERR_TIMEOUT", -9: "PP_ERR_INJECTED"}
+SHELL_PLAIN, SHELL_PERSISTENT = 0, 1
+
+
+class PPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Peaks(C.Structure):
+    _fields_ = [("capacity", C.c_int64), ("count", C.c_int64), ("needed", C.c_int64),
+                ("i", C.c_void_p), ("j", C.c_void_p), ("k", C.c_void_p), ("filt", C.c_void_p),
+                ("delta_s", C.c_void_p), ("lapd_s", C.c_void_p)]
+
+
+class Fields(C.Structure):
+    _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int),
+                ("delta", C.c_void_p), ("etax", C.c_void_p), ("etay", C.c_void_p), ("etaz", C.c_void_p),
+                ("eta2x", C.c_void_p), ("eta2y", C.c_void_p), ("eta2z", C.c_void_p),
+                ("lapd", C.c_void_p), ("on_device", C.c_int)]
+
+
+class PeakBatch(C.Structure):
+    _fields_ = [("npeaks", C.c_int64), ("i", C.c_void_p), ("j", C.c_void_p), ("k", C.c_void_p),
+                ("Rf", C.c_void_p), ("ir2min", C.c_void_p), ("ZZon", C.c_void_p), ("fcrit", C.c_void_p)]
+
+
+class Results(C.Structure):
+    _fields_ = [("RTHL", C.c_void_p), ("Srb", C.c_void_p), ("Sbar", C.c_void_p), ("Sbar2", C.c_void_p),
+                ("strain", C.c_void_p), ("gradpk", C.c_void_p), ("gradpkrf", C.c_void_p),
+                ("Fbarx", C.c_void_p), ("e_v", C.c_void_p), ("p_v", C.c_void_p),
+                ("zvir_half", C.c_void_p), ("gradpkf", C.c_void_p), ("d2F", C.c_void_p),
+                ("status", C.c_void_p), ("nshells", C.c_void_p), ("dump_counts", C.c_int64 * 6)]
+
+
+_vp, _i, _d, _f, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int64
+
+SIGNATURES = {
+    "pp_last_error": (C.c_char_p, []),
+    "pp_version": (_i, []),
+    "pp_device_count": (_i, [C.POINTER(_i)]),
+    "pp_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "pp_ctx_destroy": (_i, [_vp]),
+    "pp_ctx_synchronize": (_i, [_vp]),
+    "pp_ctx_mem_info": (_i, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "pp_upload_collapse_table": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d]),
+    "pp_build_shell_tables": (_i, [_vp, _i]),
+    "pp_shell_table_info": (_i, [_vp, C.POINTER(_i64), C.POINTER(C.c_int32)]),
+    "pp_window_multiply": (_i, [_vp, _vp, _vp, _i, _i, _d, _d, _i, _f]),
+    "pp_smooth_field": (_i, [_vp, _vp, _vp, _i, _i, _d, _d, _i]),
+    "pp_rfft": (_i, [_vp, _vp, _vp, _i, _i]),
+    "pp_find_peaks_field": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _i64, C.POINTER(Peaks)]),
+    "pp_find_peaks": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _i, _i, _i, _i, _i64, _vp, C.POINTER(Peaks)]),
+    "pp_analyse_peaks": (_i, [_vp, C.POINTER(Fields), C.POINTER(PeakBatch), _d, _d, _i, _vp, _i, _i,
+                              C.POINTER(Results)]),
+    "pp_shell_profiles": (_i, [_vp, C.POINTER(Fields), C.POINTER(PeakBatch), _d, _d, _i, _i, _vp, _vp, _vp]),
+    "pp_autotune_shell": (_i, [_vp, C.POINTER(Fields), C.POINTER(PeakBatch), _d, _d, C.POINTER(_i)]),
+    "pp_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built; run `python peakpatch.jl_amd/build.py`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != PP_OK:
+        msg = load().pp_last_error()
+        raise PPError(rc, msg.decode() if msg else "")
+    return rc

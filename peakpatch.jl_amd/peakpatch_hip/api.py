@@ -1,0 +1,256 @@
+"""Host-side front end of the C-ABI: one ``Context`` per GPU.
+
+Arrays follow the reference's memory layout: a Julia ``Array{Float32,3}`` of
+size (n, n, n) is x-fastest, so the numpy/torch view is C-order with shape
+(n, n, n) indexed [k, j, i].  Grid indices returned here are 0-based.
+
+Inputs may be numpy arrays (host) or torch CUDA tensors (device memory of the
+context's GPU); outputs of the peak and shell entry points are numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+RESULT_FIELDS = ("RTHL", "Srb", "Sbar", "Sbar2", "strain", "gradpk", "gradpkrf", "Fbarx", "e_v", "p_v",
+                 "zvir_half", "gradpkf", "d2F")
+_VEC3 = {"Sbar", "Sbar2", "gradpk", "gradpkrf", "gradpkf"}
+DUMP_NAMES = ("none", "premask", "no_fcrit", "m0_one", "no_collapse", "rthl_neg", "nonfinite")
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a, dtype, what: str):
+    """Return (address, on_device, keepalive) for a numpy array or torch tensor."""
+    if a is None:
+        return None, 0, None
+    if _is_torch(a):
+        import torch
+        want = {np.float32: torch.float32, np.int8: torch.int8, np.complex64: torch.complex64}[dtype]
+        if a.dtype != want or not a.is_contiguous():
+            raise ValueError(f"{what}: torch tensor must be contiguous {want}")
+        return a.data_ptr(), 1 if a.is_cuda else 0, a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr.ctypes.data, 0, arr
+
+
+@dataclass
+class PeakList:
+    """Output of the peak finder: 0-based indices plus per-peak filter data."""
+    i: np.ndarray
+    j: np.ndarray
+    k: np.ndarray
+    filt: np.ndarray
+    delta_s: np.ndarray
+    lapd_s: np.ndarray | None
+
+    def __len__(self):
+        return len(self.i)
+
+
+class Context:
+    """A pp_ctx: device, stream, rocFFT plans, shell/collapse tables."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        L.check(self.lib.pp_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+        self.rmax = None
+
+    def close(self):
+        if self.h:
+            self.lib.pp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------ tables
+    def upload_collapse_table(self, table: np.ndarray, X1, X2, Y1, Y2, Z1, Z2):
+        """table: (Nx, Ny, Nz) array indexed [ix, iy, iz] (x fastest in memory)."""
+        t = np.asarray(table, np.float32)
+        nx, ny, nz = t.shape
+        flat = np.ascontiguousarray(t.ravel(order="F"))
+        L.check(self.lib.pp_upload_collapse_table(self.h, flat.ctypes.data, nx, ny, nz,
+                                                  X1, X2, Y1, Y2, Z1, Z2))
+
+    def build_shell_tables(self, rmax: int):
+        L.check(self.lib.pp_build_shell_tables(self.h, int(rmax)))
+        self.rmax = int(rmax)
+
+    def shell_table_info(self):
+        nc = C.c_int64(); ns = C.c_int32()
+        L.check(self.lib.pp_shell_table_info(self.h, C.byref(nc), C.byref(ns)))
+        return nc.value, ns.value
+
+    def mem_info(self):
+        f = C.c_size_t(); t = C.c_size_t()
+        L.check(self.lib.pp_ctx_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def synchronize(self):
+        L.check(self.lib.pp_ctx_synchronize(self.h))
+
+    def last_timing(self):
+        g = C.c_double(); s = C.c_double(); p = C.c_double()
+        L.check(self.lib.pp_last_timing(self.h, C.byref(g), C.byref(s), C.byref(p)))
+        return {"gather_ms": g.value, "shell_ms": s.value, "peakfind_ms": p.value}
+
+    # ------------------------------------------------------- filter bank
+    def window_multiply(self, delta_k, n: int, boxsize: float, Rf: float, wsmooth: int, scale: float = 1.0):
+        """k-space multiply of smooth_field on a host complex64 half-spectrum
+        shaped (n, n, n//2+1); returns a new numpy array."""
+        a = np.ascontiguousarray(delta_k, np.complex64)
+        out = np.empty_like(a)
+        L.check(self.lib.pp_window_multiply(self.h, a.ctypes.data, out.ctypes.data, 0, n, boxsize, Rf,
+                                            wsmooth, scale))
+        return out
+
+    def rfft(self, field) -> np.ndarray:
+        f = np.ascontiguousarray(field, np.float32)
+        n = f.shape[0]
+        out = np.empty((n, n, n // 2 + 1), np.complex64)
+        L.check(self.lib.pp_rfft(self.h, f.ctypes.data, out.ctypes.data, 0, n))
+        return out
+
+    def smooth_field(self, delta_k, n: int, boxsize: float, Rf: float, wsmooth: int) -> np.ndarray:
+        """smooth_field (Filters.jl:159-186) on the GPU; host in, host out."""
+        a = np.ascontiguousarray(delta_k, np.complex64)
+        out = np.empty((n, n, n), np.float32)
+        L.check(self.lib.pp_smooth_field(self.h, a.ctypes.data, out.ctypes.data, 0, n, boxsize, Rf, wsmooth))
+        return out
+
+    def find_peaks_field(self, delta_s, mask, nbuff: int, fcrit: float, max_peaks: int = 2_000_000,
+                         capacity: int | None = None) -> PeakList:
+        """find_peaks (PeakFind.jl:15-78) on one field; mask (int8, same
+        shape) is updated in place.  Host numpy or device torch inputs."""
+        dp, dev, keep1 = _ptr(delta_s, np.float32, "delta_s")
+        if not _is_torch(mask):
+            if not (isinstance(mask, np.ndarray) and mask.dtype == np.int8 and mask.flags.c_contiguous):
+                raise ValueError("mask must be a C-contiguous int8 numpy array (updated in place)")
+            mp = mask.ctypes.data
+        else:
+            mp, _, _ = _ptr(mask, np.int8, "mask")
+        n3, n2, n1 = delta_s.shape
+        cap = capacity if capacity is not None else max(1, min(max_peaks, (n1 - 2 * nbuff) * (n2 - 2 * nbuff) * (n3 - 2 * nbuff)))
+        return self._run_peaks(lambda pk: self.lib.pp_find_peaks_field(
+            self.h, dp, mp, dev, n1, n2, n3, nbuff, fcrit, max_peaks, C.byref(pk)), cap, False)
+
+    def find_peaks(self, delta, boxsize: float, Rf, fcrit, wsmooth: int = 1, want_lapd: bool = False,
+                   nbuff: int = 1, max_peaks_per_filter: int = 2_000_000, mask=None,
+                   capacity: int | None = None) -> PeakList:
+        """Filter bank + stencil over all filters in the given order
+        (peak_find_tile_gpu, ext/CUDAExt.jl:3068-3197)."""
+        dp, dev, keep = _ptr(delta, np.float32, "delta")
+        n = delta.shape[0]
+        Rf = np.ascontiguousarray(Rf, np.float64)
+        fc = np.ascontiguousarray(np.broadcast_to(np.asarray(fcrit, np.float32), Rf.shape), np.float32)
+        mp = None
+        if mask is not None:
+            if not (isinstance(mask, np.ndarray) and mask.dtype == np.int8 and mask.flags.c_contiguous):
+                raise ValueError("mask must be a C-contiguous int8 numpy array (updated in place)")
+            mp = mask.ctypes.data
+        inner = max(0, n - 2 * nbuff) ** 3
+        cap = capacity if capacity is not None else max(1, min(inner, len(Rf) * max_peaks_per_filter, 1 << 24))
+        return self._run_peaks(lambda pk: self.lib.pp_find_peaks(
+            self.h, dp, dev, n, boxsize, Rf.ctypes.data, fc.ctypes.data, len(Rf), wsmooth, int(want_lapd),
+            nbuff, max_peaks_per_filter, mp, C.byref(pk)), cap, want_lapd)
+
+    def _run_peaks(self, call, cap, want_lapd) -> PeakList:
+        bufs = {k: np.zeros(cap, np.int32) for k in ("i", "j", "k", "filt")}
+        bufs["delta_s"] = np.zeros(cap, np.float32)
+        bufs["lapd_s"] = np.zeros(cap, np.float32) if want_lapd else None
+        pk = L.Peaks(cap, 0, 0, *(bufs[k].ctypes.data if bufs[k] is not None else None
+                                  for k in ("i", "j", "k", "filt", "delta_s", "lapd_s")))
+        L.check(call(pk))
+        c = pk.count
+        return PeakList(bufs["i"][:c].copy(), bufs["j"][:c].copy(), bufs["k"][:c].copy(),
+                        bufs["filt"][:c].copy(), bufs["delta_s"][:c].copy(),
+                        None if bufs["lapd_s"] is None else bufs["lapd_s"][:c].copy())
+
+    # ----------------------------------------------------- shell analysis
+    def _fields(self, delta, eta, eta2=None, lapd=None):
+        keep = []
+        ptrs = []
+        on_dev = None
+        arrs = [delta, *eta, *(eta2 if eta2 is not None else (None, None, None)), lapd]
+        for a in arrs:
+            p, dev, k = _ptr(a, np.float32, "field")
+            if a is not None:
+                on_dev = dev if on_dev is None else on_dev
+                if dev != on_dev:
+                    raise ValueError("fields must be all host or all device")
+            ptrs.append(p)
+            keep.append(k)
+        n3, n2, n1 = delta.shape
+        return L.Fields(n1, n2, n3, *ptrs, on_dev or 0), keep
+
+    @staticmethod
+    def _batch(i, j, k, Rf, ir2min, ZZon, fcrit):
+        n = len(i)
+        arrs = [np.ascontiguousarray(i, np.int32), np.ascontiguousarray(j, np.int32),
+                np.ascontiguousarray(k, np.int32),
+                np.ascontiguousarray(np.broadcast_to(Rf, (n,)), np.float64),
+                np.ascontiguousarray(np.broadcast_to(ir2min, (n,)), np.int64),
+                np.ascontiguousarray(np.broadcast_to(ZZon, (n,)), np.float64),
+                np.ascontiguousarray(np.broadcast_to(fcrit, (n,)), np.float64)]
+        return L.PeakBatch(n, *(a.ctypes.data for a in arrs)), arrs
+
+    def analyse_peaks(self, delta, eta, i, j, k, Rf, ir2min, ZZon, fcrit, alatt: float, eta2=None,
+                      lapd=None, rmax2rs: float = 0.0, nbuff: int = 0, mask=None,
+                      kernel: int = L.SHELL_PLAIN) -> dict:
+        """analyse_peak over a batch (RadialShell.jl:282-891).  Returns a dict
+        of numpy arrays keyed as PeakResult plus 'status', 'nshells',
+        'dump_counts'.  mask (int8 numpy or device tensor) is updated."""
+        f, keep = self._fields(delta, eta, eta2, lapd)
+        b, keepb = self._batch(i, j, k, Rf, ir2min, ZZon, fcrit)
+        n = len(keepb[0])
+        out = {name: np.zeros((n, 3) if name in _VEC3 else ((n, 3, 3) if name == "strain" else n), np.float64)
+               for name in RESULT_FIELDS}
+        out["status"] = np.zeros(n, np.int32)
+        out["nshells"] = np.zeros(n, np.int32)
+        r = L.Results(*(out[name].ctypes.data for name in RESULT_FIELDS), out["status"].ctypes.data,
+                      out["nshells"].ctypes.data)
+        mp, mdev = None, 0
+        if mask is not None:
+            if _is_torch(mask):
+                mp, mdev, _ = _ptr(mask, np.int8, "mask")
+            else:
+                if not (isinstance(mask, np.ndarray) and mask.dtype == np.int8 and mask.flags.c_contiguous):
+                    raise ValueError("mask must be a C-contiguous int8 numpy array (updated in place)")
+                mp = mask.ctypes.data
+        L.check(self.lib.pp_analyse_peaks(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, nbuff, mp, mdev,
+                                          kernel, C.byref(r)))
+        # strain was written column-major per peak: [p, L + 3K] -> (p, L, K)
+        out["strain"] = out["strain"].reshape(n, 9).reshape(n, 3, 3, order="F")
+        out["dump_counts"] = dict(zip(DUMP_NAMES[:6], list(r.dump_counts)))
+        return out
+
+    def shell_profiles(self, delta, eta, i, j, k, Rf, alatt: float, eta2=None, lapd=None,
+                       rmax2rs: float = 0.0, max_shells: int = 64, kernel: int = L.SHELL_PLAIN):
+        f, keep = self._fields(delta, eta, eta2, lapd)
+        b, keepb = self._batch(i, j, k, Rf, 0, 1.0, 1.0)
+        n = len(keepb[0])
+        m = np.zeros(n, np.int32)
+        ns = np.zeros((n, max_shells), np.int32)
+        fb = np.zeros((n, max_shells), np.float64)
+        L.check(self.lib.pp_shell_profiles(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, kernel, max_shells,
+                                           m.ctypes.data, ns.ctypes.data, fb.ctypes.data))
+        return m, ns, fb

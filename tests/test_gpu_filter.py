@@ -1,0 +1,156 @@
+"""GPU parity of the filter bank and the peak stencil against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import ipp1
+from oracle import oracle as O
+from peakpatch_hip import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ulp_diff(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    return np.abs(ai - bi)
+
+
+@pytest.mark.parametrize("wsmooth", [0, 1, 3])
+@pytest.mark.parametrize("n,box,Rf", [(32, 40.0, 3.7), (48, 75.0, 1.5)])
+def test_window_multiply_vs_oracle(ctx, wsmooth, n, box, Rf):
+    rng = np.random.default_rng(n + wsmooth)
+    dk = (rng.normal(size=(n, n, n // 2 + 1)) + 1j * rng.normal(size=(n, n, n // 2 + 1))).astype(np.complex64)
+    g = ctx.window_multiply(dk, n, box, Rf, wsmooth)
+    o = O.window_multiply(dk, n, box, Rf, wsmooth)
+    gv, ov = g.view(np.float32), o.view(np.float32)
+    u = _ulp_diff(gv, ov)
+    # |k| and W are Float64 on both sides; only libm last bits can move the
+    # Float32 rounding of W.
+    assert u.max() <= 1
+    assert (u == 0).mean() > 0.999
+    assert g[0, 0, 0] == dk[0, 0, 0]
+
+
+@pytest.mark.parametrize("wsmooth", [0, 1, 3])
+def test_smooth_field_vs_fp64(ctx, wsmooth):
+    n, box, Rf = 64, 75.0, 2.5
+    t = synth.make_tile(n, box, seed=7, ilpt=1, want_lapd=False)
+    g = ctx.smooth_field(t.delta_k, n, box, Rf, wsmooth)
+    o = O.smooth_field(t.delta_k, n, box, Rf, wsmooth)
+    scale = np.abs(o).max()
+    assert np.abs(g - o).max() <= 2e-6 * scale
+
+
+def test_rfft_vs_numpy(ctx):
+    n = 32
+    f = np.random.default_rng(1).normal(size=(n, n, n)).astype(np.float32)
+    g = ctx.rfft(f)
+    o = np.fft.rfftn(f.astype(np.float64), axes=(0, 1, 2))
+    assert np.abs(g - o).max() <= 1e-5 * np.abs(o).max()
+
+
+# -------------------------------------------------------------- stencil
+def _gpu_fp(ctx, d, mask=None, nbuff=1, fcrit=1.0, max_peaks=2_000_000):
+    mask = np.zeros(d.shape, np.int8) if mask is None else mask
+    pk = ctx.find_peaks_field(d, mask, nbuff, fcrit, max_peaks)
+    return pk, mask
+
+
+def test_find_peaks_known_answers(ctx):
+    # test/test_peakfind.jl:4-118 on the GPU stencil (0-based indices here)
+    z = lambda n: np.zeros((n, n, n), np.float32)
+    d = z(11); d[5, 5, 5] = 2.0
+    pk, m = _gpu_fp(ctx, d)
+    assert list(zip(pk.i, pk.j, pk.k)) == [(5, 5, 5)] and pk.delta_s[0] == 2.0 and m[5, 5, 5] == 1
+    assert len(_gpu_fp(ctx, np.full((11, 11, 11), 0.5, np.float32))[0]) == 0
+    g = np.fromfunction(lambda k, j, i: i + j + k + 3, (11, 11, 11)).astype(np.float32)
+    assert len(_gpu_fp(ctx, g)[0]) == 0
+    d = z(11); d[5, 5, 5] = 2.0; mask = np.zeros_like(d, np.int8); mask[5, 5, 5] = 1
+    assert len(_gpu_fp(ctx, d, mask)[0]) == 0
+    d = z(11); d[5, 5, 0] = 2.0
+    assert len(_gpu_fp(ctx, d)[0]) == 0
+    d = z(15); d[3, 3, 3] = 2.0; d[11, 11, 11] = 3.0
+    pk, _ = _gpu_fp(ctx, d)
+    assert list(zip(pk.i, pk.j, pk.k)) == [(3, 3, 3), (11, 11, 11)]
+    d = z(11); d[6, 4, 2] = 2.0
+    pk, _ = _gpu_fp(ctx, d)
+    assert ipp1(pk.i[0], pk.j[0], pk.k[0], 11) == 3 + 4 * 11 + 6 * 121
+    d = z(15)
+    for (a, b, c) in [(4, 4, 4), (4, 4, 12), (12, 12, 4), (12, 12, 12)]:
+        d[c - 1, b - 1, a - 1] = 2.0
+    pk, m = _gpu_fp(ctx, d, max_peaks=2)
+    assert len(pk) == 2 and m.sum() == 2
+    assert list(zip(pk.i, pk.j, pk.k)) == [(3, 3, 3), (11, 11, 3)]
+
+
+@pytest.mark.parametrize("nbuff", [1, 3])
+@pytest.mark.parametrize("shape", [(40, 40, 40), (33, 41, 37)])
+def test_find_peaks_field_bitexact(ctx, nbuff, shape):
+    rng = np.random.default_rng(sum(shape) + nbuff)
+    d = rng.normal(size=shape).astype(np.float32)
+    d[10:13, 10:13, 10:13] = 1.7      # plateau: ties allowed
+    mask = (rng.random(shape) < 0.05).astype(np.int8)
+    m_cpu = mask.copy()
+    i, j, k, v = O.find_peaks(d, m_cpu, nbuff, 0.3)
+    pk, m_gpu = _gpu_fp(ctx, d, mask.copy(), nbuff, 0.3)
+    assert np.array_equal(pk.i + 1, i) and np.array_equal(pk.j + 1, j) and np.array_equal(pk.k + 1, k)
+    assert np.array_equal(pk.delta_s, v)
+    assert np.array_equal(m_gpu, m_cpu)
+
+
+def test_find_peaks_max_peaks_cap_matches_cpu_early_return(ctx):
+    d = np.random.default_rng(9).normal(size=(36, 36, 36)).astype(np.float32)
+    for cap in (1, 7, 100):
+        mc = np.zeros(d.shape, np.int8)
+        i, j, k, _ = O.find_peaks(d, mc, 1, 0.0, cap)
+        pk, mg = _gpu_fp(ctx, d, None, 1, 0.0, cap)
+        assert len(pk) == len(i) == min(cap, len(pk)) and np.array_equal(pk.i + 1, i)
+        assert np.array_equal(mg, mc)
+
+
+def test_find_peaks_capacity_error(ctx):
+    from peakpatch_hip import PPError
+    d = np.random.default_rng(2).normal(size=(24, 24, 24)).astype(np.float32)
+    with pytest.raises(PPError, match="CAPACITY"):
+        ctx.find_peaks_field(d, np.zeros(d.shape, np.int8), 1, 0.0, capacity=3)
+
+
+def test_find_peaks_empty_inner_cube(ctx):
+    d = np.ones((8, 8, 8), np.float32)
+    assert len(ctx.find_peaks_field(d, np.zeros(d.shape, np.int8), 4, 0.0)) == 0
+
+
+def test_filter_bank_vs_oracle(ctx):
+    """peak_find_tile_gpu semantics: filters in descending Rf, shared mask,
+    fcrit per filter; peak lists compared with the oracle run on the same
+    delta_k (FFTW/pocketfft vs rocFFT last bits can only flip exact ties)."""
+    n, box, nbuff = 64, 75.0, 4
+    t = synth.make_tile(n, box, seed=42, ilpt=1, want_lapd=False)
+    Rf = synth.filter_bank(6, 1.5, 8.0)
+    fcrit = 1.686
+    mask_g = np.zeros((n, n, n), np.int8)
+    pk = ctx.find_peaks(t.delta, box, Rf, fcrit, wsmooth=1, want_lapd=True, nbuff=nbuff, mask=mask_g)
+    mask_c = np.zeros((n, n, n), np.int8)
+    ref = []
+    for f, R in enumerate(Rf):
+        s = O.smooth_field(t.delta_k, n, box, R, 1).astype(np.float32)
+        i, j, k, v = O.find_peaks(s, mask_c, nbuff, fcrit)
+        lap = O.smooth_field(t.delta_k, n, box, R, 3)
+        ref += [(f, a - 1, b - 1, c - 1, vv, lap[c - 1, b - 1, a - 1]) for a, b, c, vv in zip(i, j, k, v)]
+    got = list(zip(pk.filt.tolist(), pk.i.tolist(), pk.j.tolist(), pk.k.tolist()))
+    exp = [tuple(int(x) for x in r[:4]) for r in ref]
+    assert len(got) > 50
+    # FP32 rocFFT vs FP64 pocketfft can only flip exact near-ties; allow at
+    # most 0.5% of the list to differ and require the order of the common
+    # peaks to agree.
+    diff = set(got) ^ set(exp)
+    assert len(diff) <= max(2, len(exp) // 200), diff
+    common = set(got) & set(exp)
+    assert [g for g in got if g in common] == [e for e in exp if e in common]
+    assert (mask_g != mask_c).sum() <= len(diff)
+    idx = {e: q for q, e in enumerate(exp)}
+    sel = [(q, idx[g]) for q, g in enumerate(got) if g in idx]
+    gq, eq = np.array(sel).T
+    ev = np.array([r[4] for r in ref]); el = np.array([r[5] for r in ref])
+    assert np.allclose(pk.delta_s[gq], ev[eq], rtol=1e-5, atol=1e-5)
+    assert np.allclose(pk.lapd_s[gq], el[eq], rtol=1e-4, atol=1e-4 * np.abs(el).max())
