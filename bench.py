@@ -1,0 +1,205 @@
+#!/usr/bin/env python
+"""Benchmark of the halo-finding hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
+
+Workload (default c3): one light-cone tile, nmesh=512, box 600 Mpc/h, 2LPT,
+ioutshear=1, 25 log-spaced top-hat filters 1-30 Mpc/h, rmax2rs=2, observer at
+the box corner, seed 42.  Set-up (untimed): the fields are synthesised on the
+GPU, the filter bank + stencil finds the peaks, the host computes per-peak
+redshift / fcrit.  A step is one shell analysis of every peak of the tile
+(gather + virialisation walk + R_TH + mask), fields resident in HBM.
+
+Prints one JSON line: shell-analysed peaks/s, the roofline of the shell
+kernel (algorithmic bytes = 4 B x F fields x cells gathered), and the CPU
+baseline (the FP64 restatement of the reference algorithm, OpenMP over peaks)
+timed on a bounded stratified sample of the same peaks.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "peakpatch.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "shell-analysed peaks/s at nmesh=512 + % HBM peak; p50/p99 tile latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def nfields(cfg) -> int:
+    return 1 + 1 + 3 + (3 if cfg.ilpt >= 2 else 0) + (1 if cfg.ioutshear >= 1 else 0)
+
+
+def load_table(ctx):
+    from peakpatch_hip import collapse
+    path = os.path.join(ROOT, "tests", "golden", "HomelTab_fortran.dat")
+    ct = collapse.read_homeltab(path) if os.path.exists(path) else collapse.sinusoidal_table()
+    ctx.upload_collapse_table(ct.table, *ct.bounds)
+    return ct, os.path.basename(path) if os.path.exists(path) else "sinusoidal"
+
+
+def time_form(drv, ctx, fields, b, mask, form, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        mask.zero_()
+        torch.cuda.synchronize()
+        drv.analyse(ctx, fields, b, mask=mask, kernel=form)
+    walls, kern = [], []
+    res = None
+    for _ in range(steps):
+        mask.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = drv.analyse(ctx, fields, b, mask=mask, kernel=form)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        kern.append(ctx.last_timing()["shell_ms"] / 1e3)
+    return res, walls, kern
+
+
+def cpu_baseline(cfg, tile, b, ct, seconds: float):
+    """Oracle (FP64 C restatement, OpenMP dynamic over peaks) on a stratified
+    sample of the tile's peaks, sized to about `seconds` of CPU time."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    host = lambda t: t.detach().cpu().numpy()
+    delta = host(tile.delta)
+    eta = tuple(host(a) for a in tile.eta)
+    eta2 = tuple(host(a) for a in tile.eta2) if tile.eta2 is not None else None
+    lapd = host(tile.lapd) if tile.lapd is not None else None
+    mask = np.zeros(delta.shape, np.int8)
+    grid = O.Grid(delta, *eta, eta2=eta2, mask=mask, lapd=lapd)
+    tab = O.Table(ct.table, *ct.bounds)
+    sh = O.Shells(cfg.nhunt)
+    n = cfg.n
+    ipp = (b.i.astype(np.int64) + 1) + b.j.astype(np.int64) * n + b.k.astype(np.int64) * n * n
+
+    def run(sel):
+        t0 = time.perf_counter()
+        O.analyse_peaks(grid, ipp[sel], b.ir2min[sel], b.ZZon[sel], b.Rf[sel], b.fcrit[sel], cfg.alatt, tab, sh,
+                        cfg.nbuff_eff, cfg.rmax2rs, threads)
+        return time.perf_counter() - t0
+
+    N = len(ipp)
+    stride = max(1, N // 2000)
+    t = run(np.arange(0, N, stride))
+    est_all = t * stride
+    stride2 = max(1, int(math.ceil(est_all / seconds)))
+    sel = np.arange(0, N, stride2)
+    t = run(sel)
+    return {"value": len(sel) / t, "unit": "peaks/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride2}th peak of the same tile ({len(sel)} of {N}, all filters), "
+                      f"{t:.1f} s, FP64 oracle restating RadialShell.analyse_peak"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--form", default="auto", choices=["auto", "plain", "persistent"])
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-peak form")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(tiles would shard one per GPU; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from peakpatch_hip import SHELL_PERSISTENT, SHELL_PLAIN, Context, PPError, synth
+    from peakpatch_hip.pipeline import CONFIGS, TileDriver
+
+    cfg = CONFIGS[args.config]
+    ctx = Context(0)
+    ct, table_name = load_table(ctx)
+    ctx.build_shell_tables(cfg.nhunt)
+    t0 = time.perf_counter()
+    tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1,
+                                 device="cuda")
+    torch.cuda.synchronize()
+    log(f"[bench] {cfg.describe()}; fields synthesised in {time.perf_counter() - t0:.1f} s")
+    drv = TileDriver(cfg)
+    t0 = time.perf_counter()
+    peaks = drv.find_peaks(ctx, tile.delta)
+    pf_wall = time.perf_counter() - t0
+    pf_ms = ctx.last_timing()["peakfind_ms"]
+    b = drv.batch(peaks)
+    log(f"[bench] filter bank + stencil: {len(peaks)} peaks, {pf_ms:.1f} ms on device ({pf_wall:.2f} s wall)")
+    fields = (tile.delta, tile.eta, tile.eta2, tile.lapd)
+    mask = torch.zeros((cfg.n,) * 3, dtype=torch.int8, device="cuda")
+
+    forms = []
+    if args.form in ("auto", "persistent"):
+        forms.append(("persistent", SHELL_PERSISTENT))
+    if args.form == "plain" or (args.form == "auto" and not args.no_plain):
+        forms.append(("plain", SHELL_PLAIN))
+    timings = {}
+    results = {}
+    for name, form in forms:
+        try:
+            res, walls, kern = time_form(drv, ctx, fields, b, mask, form, args.steps, args.warmup)
+        except PPError as e:
+            log(f"[bench] {name} form unavailable: {e}")
+            continue
+        timings[name] = (walls, kern)
+        results[name] = res
+        log(f"[bench] {name}: wall {np.median(walls) * 1e3:.1f} ms/step, kernel {np.median(kern) * 1e3:.1f} ms, "
+            f"collapsed {(res['RTHL'] > 0).sum()}, dumps {res['dump_counts']}")
+    head = "persistent" if "persistent" in timings else "plain"
+    walls, kern = timings[head]
+    npk = len(b)
+    ms_step = float(np.mean(walls)) * 1e3
+    k_s = float(np.mean(kern))
+    cells = drv.gather_cells(b)
+    F = nfields(cfg)
+    alg_bytes = float(4 * F * cells.sum())
+    achieved = alg_bytes / k_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": npk / (ms_step / 1e3),
+        "unit": "peaks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg.describe(), "npeaks": npk, "kernel_form": head,
+                   "collapse_table": table_name, "fields_F": F, "cells_gathered": int(cells.sum())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernel_ms_per_step": k_s * 1e3,
+        "kernel_peaks_per_s": npk / k_s,
+        "peakfind_ms": pf_ms,
+        "halos": int((results[head]["RTHL"] > 0).sum()),
+    }
+    if "plain" in timings and head != "plain":
+        pw, pk = timings["plain"]
+        out["plain_kernel_ms"] = float(np.mean(pk)) * 1e3
+        out["plain_peaks_per_s"] = npk / float(np.mean(pw))
+        out["speedup_vs_plain_kernel"] = float(np.mean(pk)) / k_s
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, tile, b, ct, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -411,6 +411,10 @@ double orc_dlinear_ab(const orc_dlin *T, double ai, double *HD_Ha, double *D_a)
         int nlint, kkl;
         if (kk < 2 || kk >= ntab) { nlint = 2; kkl = kk - 1 > 1 ? kk - 1 : 1; }
         else { nlint = 4; kkl = kk - 2; }
+        /* kk == 2 makes the reference index atab[0] (a BoundsError at
+         * z ~ 1000); use the two-point rule there instead of reading out of
+         * bounds. */
+        if (kkl < 1) { nlint = 2; kkl = 1; }
         const double *xa = &T->atab[kkl - 1], *v1 = &T->D_atab[kkl - 1], *v2 = &T->dlnD[kkl - 1];
         double x = ai;
         if (nlint == 4) {

@@ -395,7 +395,12 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     const int n = fields->n1;
     std::vector<int8_t> mask_host_copy;
     const int8_t *mask_for_params = nullptr;
-    if (mask && rmax2rs > 0.0) {
+    // the pre-mask test can only fire where floor(Rf / alatt * rmax2rs) == 0
+    bool premask_possible = false;
+    if (mask && rmax2rs > 0.0)
+        for (int64_t p = 0; p < np && !premask_possible; p++)
+            premask_possible = std::floor(peaks->Rf[p] / alatt * rmax2rs) < 1.0;
+    if (premask_possible) {
         if (mask_on_device) {
             mask_host_copy.resize((size_t)n * n * n);
             PP_HIP(hipMemcpy(mask_host_copy.data(), mask, mask_host_copy.size(), hipMemcpyDeviceToHost));

@@ -244,3 +244,131 @@ def make_tile(n: int, boxsize: float, seed: int = 42, ilpt: int = 2, want_lapd: 
     eta2 = displacements_2lpt(delta_k, n, boxsize, workers) if ilpt >= 2 else None
     lapd = compute_laplacian(delta_k, n, boxsize, workers) if want_lapd else None
     return TileInputs(n, boxsize, delta, delta_k, eta, eta2, lapd)
+
+
+# ---------------------------------------------------------------------------
+# Device (torch) version of the same producers, for bench-size tiles.  Same
+# formulas as above; Threefry runs on int64 tensors (two's-complement adds,
+# masked arithmetic shifts give the logical rotations).
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _t_rotl(x, r: int):
+    return (x << r) | ((x >> (64 - r)) & ((1 << r) - 1))
+
+
+def threefry2x64_20_torch(ctr0, key0: int):
+    """Threefry-2x64-20 of counters (ctr0, 0) under key (key0, 0)."""
+    torch = _torch()
+    par = 0x1BD11BDAA9FC1A22
+    ks = (key0, 0, par ^ key0)
+    x0 = ctr0 + ks[0]
+    x1 = torch.zeros_like(ctr0) + ks[1]
+    for blk in range(5):
+        for r in range(4):
+            x0 = x0 + x1
+            x1 = _t_rotl(x1, _ROT[(4 * blk + r) % 8])
+            x1 = x1 ^ x0
+        inj = blk + 1
+        x0 = x0 + ks[inj % 3]
+        x1 = x1 + (ks[(inj + 1) % 3] + inj)
+    return x0, x1
+
+
+def fill_noise_threefry_torch(n: int, seed: int, device, chunk: int = 1 << 24):
+    torch = _torch()
+    n3 = n * n * n
+    npairs = (n3 + 1) // 2
+    out = torch.empty(2 * npairs, dtype=torch.float32, device=device)
+    m53 = (1 << 53) - 1
+    for p0 in range(0, npairs, chunk):
+        p1 = min(npairs, p0 + chunk)
+        ctr = torch.arange(p0, p1, dtype=torch.int64, device=device)
+        u1r, u2r = threefry2x64_20_torch(ctr, int(seed))
+        u1 = (((u1r >> 11) & m53).to(torch.float64) + 1.0) * 1.1102230246251565e-16
+        u2 = (((u2r >> 11) & m53).to(torch.float64) + 1.0) * 1.1102230246251565e-16
+        r = torch.sqrt(-2.0 * torch.log(u1))
+        th = 2.0 * math.pi * u2
+        seg = torch.stack([r * torch.cos(th), r * torch.sin(th)], dim=1).reshape(-1)
+        out[2 * p0:2 * p1] = seg.to(torch.float32)
+    return out[:n3].reshape(n, n, n)
+
+
+def _kgrid_torch(n, boxsize, device):
+    torch = _torch()
+    dk = 2.0 * math.pi / boxsize
+    kx = dk * torch.fft.rfftfreq(n, 1.0 / n, dtype=torch.float64, device=device)
+    ky = dk * torch.fft.fftfreq(n, 1.0 / n, dtype=torch.float64, device=device)
+    return kx[None, None, :], ky[None, :, None], ky[:, None, None]
+
+
+def make_tile_torch(n: int, boxsize: float, seed: int = 42, ilpt: int = 2, want_lapd: bool = True,
+                    pk=None, device="cuda"):
+    """make_tile on a torch device; returns TileInputs holding torch tensors
+    (delta_k stays on the device as complex64)."""
+    torch = _torch()
+    pk = pk or EH98NoWiggle()
+    noise = fill_noise_threefry_torch(n, seed, device)
+    kx, ky, kz = _kgrid_torch(n, boxsize, device)
+    k2 = kx * kx + ky * ky + kz * kz
+    dk = 2.0 * math.pi / boxsize
+    kmag = torch.sqrt(k2)
+    # P(k) evaluated on the host over the distinct |k| would be exact; the
+    # device evaluates the same closed form in float64.
+    amp = torch.sqrt(_pk_torch(pk, kmag) * dk ** 3 * float(n) ** 3)
+    amp[0, 0, 0] = 0.0
+    nk = torch.fft.rfftn(noise)
+    del noise
+    nk = nk * amp.to(torch.float32)
+    delta = torch.fft.irfftn(nk, s=(n, n, n)).to(torch.float32).contiguous()
+    del nk, amp
+    delta_k = torch.fft.rfftn(delta)
+    k2s = k2.clone()
+    k2s[0, 0, 0] = 1.0
+    keep = torch.ones(delta_k.shape, dtype=torch.bool, device=device)
+    keep[:, :, -1] = False
+    keep[:, n // 2, :] = False
+    keep[n // 2, :, :] = False
+    keep[0, 0, 0] = False
+    eta = []
+    for kc in (kx, ky, kz):
+        m = (kc / k2s).to(torch.float32)
+        eta.append(torch.fft.irfftn(torch.where(keep, 1j * m * delta_k, 0), s=(n, n, n)).contiguous())
+    eta2 = None
+    if ilpt >= 2:
+        kk = (kx, ky, kz)
+        phi = {}
+        for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)):
+            m = (-kk[a] * kk[b] / k2s).to(torch.float32)
+            t = m * delta_k
+            t[0, 0, 0] = 0
+            phi[(a, b)] = torch.fft.irfftn(t, s=(n, n, n))
+        src = (phi[(0, 0)] * phi[(1, 1)] - phi[(0, 1)] ** 2 + phi[(0, 0)] * phi[(2, 2)] - phi[(0, 2)] ** 2
+               + phi[(1, 1)] * phi[(2, 2)] - phi[(1, 2)] ** 2)
+        del phi
+        sk = torch.fft.rfftn(src)
+        del src
+        eta2 = []
+        for kc in kk:
+            m = (kc / k2s).to(torch.float32)
+            eta2.append(torch.fft.irfftn(torch.where(keep, -1j * m * sk, 0), s=(n, n, n)).contiguous())
+        del sk
+    lapd = None
+    if want_lapd:
+        lapd = (torch.fft.irfftn(k2.to(torch.float32) * delta_k, s=(n, n, n)) / float(n) ** 3).contiguous()
+    return TileInputs(n, boxsize, delta, delta_k, tuple(eta), None if eta2 is None else tuple(eta2), lapd)
+
+
+def _pk_torch(pk: EH98NoWiggle, k):
+    torch = _torch()
+    ks = k * pk.h * pk.s
+    gamma = pk.Om * pk.h * (pk.alpha + (1.0 - pk.alpha) / (1.0 + (0.43 * ks) ** 4))
+    q = k * pk.theta ** 2 / gamma
+    L0 = torch.log(2.0 * math.e + 1.8 * q)
+    C0 = 14.2 + 731.0 / (1.0 + 62.5 * q)
+    T = L0 / (L0 + C0 * q * q)
+    kk = torch.where(k > 0, k, torch.ones_like(k))
+    return torch.where(k > 0, pk.A * kk ** pk.ns * T * T / (2.0 * math.pi) ** 3, torch.zeros_like(k))
