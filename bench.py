@@ -47,23 +47,27 @@ def load_table(ctx):
     return ct, os.path.basename(path) if os.path.exists(path) else "sinusoidal"
 
 
-def time_form(drv, ctx, fields, b, mask, form, steps, warmup):
+def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
+    """Device-resident steps: batch, fields, mask and results stay in HBM."""
     import torch
     for _ in range(warmup):
         mask.zero_()
         torch.cuda.synchronize()
-        drv.analyse(ctx, fields, b, mask=mask, kernel=form)
-    walls, kern = [], []
+        drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
+    walls, kern, gath = [], [], []
     res = None
     for _ in range(steps):
         mask.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = drv.analyse(ctx, fields, b, mask=mask, kernel=form)
+        res = drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
         torch.cuda.synchronize()
         walls.append(time.perf_counter() - t0)
-        kern.append(ctx.last_timing()["shell_ms"] / 1e3)
-    return res, walls, kern
+        tm = ctx.last_timing()
+        kern.append(tm["shell_ms"] / 1e3)
+        gath.append(tm["gather_ms"] / 1e3)
+    res = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in res.items()}
+    return res, walls, kern, gath
 
 
 def cpu_baseline(cfg, tile, b, ct, seconds: float):
@@ -141,6 +145,9 @@ def main():
     log(f"[bench] filter bank + stencil: {len(peaks)} peaks, {pf_ms:.1f} ms on device ({pf_wall:.2f} s wall)")
     fields = (tile.delta, tile.eta, tile.eta2, tile.lapd)
     mask = torch.zeros((cfg.n,) * 3, dtype=torch.int8, device="cuda")
+    bdev = b.to_device()
+    prop = torch.cuda.get_device_properties(0)
+    log(f"[bench] device {prop.name}: {prop.multi_processor_count} CUs, {prop.total_memory / 2**30:.0f} GiB")
 
     forms = []
     if args.form in ("auto", "persistent"):
@@ -151,16 +158,16 @@ def main():
     results = {}
     for name, form in forms:
         try:
-            res, walls, kern = time_form(drv, ctx, fields, b, mask, form, args.steps, args.warmup)
+            res, walls, kern, gath = time_form(drv, ctx, fields, bdev, mask, form, args.steps, args.warmup)
         except PPError as e:
             log(f"[bench] {name} form unavailable: {e}")
             continue
-        timings[name] = (walls, kern)
+        timings[name] = (walls, kern, gath)
         results[name] = res
-        log(f"[bench] {name}: wall {np.median(walls) * 1e3:.1f} ms/step, kernel {np.median(kern) * 1e3:.1f} ms, "
-            f"collapsed {(res['RTHL'] > 0).sum()}, dumps {res['dump_counts']}")
+        log(f"[bench] {name}: wall {np.median(walls) * 1e3:.1f} ms/step, device {np.median(kern) * 1e3:.1f} ms "
+            f"(gather {np.median(gath) * 1e3:.1f} ms), collapsed {(res['RTHL'] > 0).sum()}, dumps {res['dump_counts']}")
     head = "persistent" if "persistent" in timings else "plain"
-    walls, kern = timings[head]
+    walls, kern, gath = timings[head]
     npk = len(b)
     ms_step = float(np.mean(walls)) * 1e3
     k_s = float(np.mean(kern))
@@ -186,12 +193,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernel_ms_per_step": k_s * 1e3,
+        "gather_ms_per_step": float(np.mean(gath)) * 1e3,
         "kernel_peaks_per_s": npk / k_s,
         "peakfind_ms": pf_ms,
         "halos": int((results[head]["RTHL"] > 0).sum()),
     }
     if "plain" in timings and head != "plain":
-        pw, pk = timings["plain"]
+        pw, pk, _ = timings["plain"]
         out["plain_kernel_ms"] = float(np.mean(pk)) * 1e3
         out["plain_peaks_per_s"] = npk / float(np.mean(pw))
         out["speedup_vs_plain_kernel"] = float(np.mean(pk)) / k_s

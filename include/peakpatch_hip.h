@@ -54,6 +54,7 @@ extern "C" {
 #define PP_DUMP_M0_ONE 3
 #define PP_DUMP_NO_COLLAPSE 4
 #define PP_DUMP_RTHL_NEG 5
+#define PP_DUMP_NONFINITE 6
 
 typedef struct pp_ctx pp_ctx;
 
@@ -149,6 +150,7 @@ typedef struct {
     const int64_t *ir2min;
     const double *ZZon;       /* 1 + z of the peak                         */
     const double *fcrit;      /* collapse threshold (fsc_of_z or override) */
+    int on_device;            /* 1: the arrays above are device memory     */
 } pp_peak_batch;
 
 /* PeakResult (RadialShell.jl:58-72) as SoA host arrays, Float64.  Vectors
@@ -157,9 +159,11 @@ typedef struct {
 typedef struct {
     double *RTHL, *Srb, *Sbar, *Sbar2, *strain, *gradpk, *gradpkrf;
     double *Fbarx, *e_v, *p_v, *zvir_half, *gradpkf, *d2F;
-    int32_t *status;          /* PP_DUMP_* per peak                        */
+    int32_t *status;          /* PP_DUMP_* per peak (6: non-finite strain,
+                                 where the reference's LAPACK call raises) */
     int32_t *nshells;         /* finalised shells per peak (diagnostic)    */
-    int64_t dump_counts[6];   /* out: per PP_DUMP_* totals (get_dump_counts) */
+    int on_device;            /* 1: the arrays above are device memory     */
+    int64_t dump_counts[7];   /* out: per status totals (get_dump_counts)  */
 } pp_results;
 
 /* analyse_peak over a batch (src/HaloFinder/RadialShell.jl:282-891 semantics;

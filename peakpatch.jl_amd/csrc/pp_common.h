@@ -123,7 +123,10 @@ struct pp_ctx {
     // scratch
     pp::DevBuf d_field_a, d_field_b, d_k0, d_k1, d_k2, d_mask, d_misc, d_peaks, d_scan, d_cub;
     pp::DevBuf d_in[8], d_pk, d_res, d_prof, d_queue;
-    // autotuned persistent launch shape: workgroups per CU
+    // persistent-form tunables (autotuned): tiny-peak threshold in finalised
+    // shells, and the cap on gather waves per CU
+    int tiny_m = 16;
+    int gather_waves_cu = 16;
     int wg_per_cu = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     pp::Timing timing;

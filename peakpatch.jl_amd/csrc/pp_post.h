@@ -49,6 +49,15 @@ struct Table3 {
     double X1, X2, Y1, Y2, Z1, Z2;
 };
 
+// Result arrays in the caller's layout (PeakResult as SoA, RadialShell.jl:58-72);
+// all device pointers, index = the caller's peak index.
+struct ResultView {
+    double *RTHL, *Srb, *Sbar, *Sbar2, *strain, *gradpk, *gradpkrf, *Fbarx, *e_v, *p_v, *zvir_half, *gradpkf, *d2F;
+    int32_t *status, *nshells;
+    int32_t *m0;  // aux: final m0 per peak (0 if no collapse), for the deferred finish pass
+    int32_t *q;   // aux: last shell with rad <= RTHL
+};
+
 struct PostConst {
     const double *rad;    // rad[s-1] = sqrt(r2 of shell s)
     const double *akk;    // atab4, 1101 entries
@@ -185,14 +194,28 @@ struct Strain {
     double g[3], gf[3];
 };
 
-// kernel_strain (RadialShell.jl:228-277), wave-parallel over m1.
-template <class PV>
+// Execution modes of the post-process: COOP = the 64 lanes of a wave work on
+// one peak (radial sums split over lanes, butterfly-combined); !COOP = every
+// lane runs its own peak serially.
+template <bool COOP> struct Exec {
+    static constexpr int W = COOP ? 64 : 1;
+    __device__ static int lane() { return COOP ? (int)(threadIdx.x & 63) : 0; }
+    __device__ static double sum(double v)
+    {
+        if constexpr (COOP) return wave_sum(v);
+        else return v;
+    }
+    __device__ static bool leader() { return COOP ? (threadIdx.x & 63) == 0 : true; }
+};
+
+// kernel_strain (RadialShell.jl:228-277).
+template <bool COOP, class PV>
 __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlow, int mupp)
 {
-    const int lane = threadIdx.x & 63;
+    using X = Exec<COOP>;
     double E[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
     const double rmp = P.rad(mp);
-    for (int m1 = mlow + lane; m1 <= mupp; m1 += 64) {
+    for (int m1 = mlow + X::lane(); m1 <= mupp; m1 += X::W) {
         double wt;
         const double r1 = P.rad(m1);
         if (mp > 1) {
@@ -210,28 +233,25 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
             double aww = C.akk[icon] + diff * (C.akk[icon + 1] - C.akk[icon]);
             wt = 0.5 * aww / r1;
         }
-        double G[3], Gf[3], SR[9];
-        P.strain_inputs(m1, G, Gf, SR);
+        double G[3], Gf[3], Es[6];
+        P.strain_inputs(m1, G, Gf, Es);
 #pragma unroll
         for (int L = 0; L < 3; L++) {
             g[L] += wt * G[L];
             gf[L] += wt * Gf[L];
         }
-        E[0] -= 0.5 * wt * (SR[0] + SR[0]);
-        E[1] -= 0.5 * wt * (SR[4] + SR[4]);
-        E[2] -= 0.5 * wt * (SR[8] + SR[8]);
-        E[3] -= 0.5 * wt * (SR[1] + SR[3]);
-        E[4] -= 0.5 * wt * (SR[2] + SR[6]);
-        E[5] -= 0.5 * wt * (SR[5] + SR[7]);
+        // Es = {SR00+SR00, SR11+SR11, SR22+SR22, SR01+SR10, SR02+SR20, SR12+SR21}
+#pragma unroll
+        for (int q = 0; q < 6; q++) E[q] -= 0.5 * wt * Es[q];
     }
     Strain s;
     const double scale = mp > 1 ? C.wRnor / rmp : C.aRnor;
 #pragma unroll
-    for (int i = 0; i < 6; i++) s.E[i] = wave_sum(E[i]) * scale;
+    for (int i = 0; i < 6; i++) s.E[i] = X::sum(E[i]) * scale;
 #pragma unroll
     for (int L = 0; L < 3; L++) {
-        s.g[L] = wave_sum(g[L]) * scale;
-        s.gf[L] = wave_sum(gf[L]) * scale;
+        s.g[L] = X::sum(g[L]) * scale;
+        s.gf[L] = X::sum(gf[L]) * scale;
     }
     return s;
 }
@@ -257,13 +277,102 @@ __device__ __forceinline__ void evals_ep(const double E[6], double &Frho, double
     }
 }
 
-// The per-peak analysis.  P provides rad(s), Fbar(s), count(s), sums of S /
-// S2 / lapd and strain_inputs(s) (G, Gf, SR already divided by rad, zero at
-// s = 1).  res: R_NREC doubles.  Returns the dump reason.
-template <class PV>
-__device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C, double *res)
+__device__ __forceinline__ void write_nocollapse(const ResultView &R, int64_t o, int status)
 {
-    const int lane = threadIdx.x & 63;
+    R.RTHL[o] = -1.0; R.Srb[o] = 0.0; R.Fbarx[o] = 0.0; R.e_v[o] = 0.0; R.p_v[o] = 0.0;
+    R.zvir_half[o] = -1.0; R.d2F[o] = 0.0;
+    for (int L = 0; L < 3; L++) {
+        R.Sbar[o * 3 + L] = 0.0; R.Sbar2[o * 3 + L] = 0.0; R.gradpk[o * 3 + L] = 0.0;
+        R.gradpkrf[o * 3 + L] = 0.0; R.gradpkf[o * 3 + L] = 0.0;
+    }
+    for (int q = 0; q < 9; q++) R.strain[o * 9 + q] = 0.0;
+    R.status[o] = status;
+    R.m0[o] = 0;
+    R.q[o] = 0;
+}
+
+// Sum of S, S2 over shells 1..m0-1 and of lapd over cells 2..(r <= RTHL)
+// with the stop at the first out-of-bounds cell, plus the mask writes
+// (RadialShell.jl:756-819), from the cell table.  Used by the finish pass;
+// COOP splits the cells over the lanes.
+template <bool COOP>
+__device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, int q,
+                             const float *const ex[3], const float *const e2[3], const ResultView &R, int64_t o)
+{
+    using X = Exec<COOP>;
+    const int cS = C.shell_start[m0 - 1];          // cells of shells 1..m0-1
+    const int cend = min(C.shell_start[q], pk.npart);
+    const long n1 = C.n1, n12 = (long)C.n1 * C.n2;
+    double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
+    const bool inside = pk.flags & 1;             // the whole gather sphere is inside the grid
+    double sL = 0.0, nL = 0.0;
+    int firstoob = 0x7fffffff;
+    const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
+    const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
+    const int cmax = max(cS, cend);
+    for (int c = X::lane(); c < cmax; c += X::W) {
+        const int pc = C.cell[c];
+        const int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+        const bool inb = inside || (x >= 0 && x < C.n1 && y >= 0 && y < C.n2 && z >= 0 && z < C.n3);
+        const long a = x + y * n1 + z * n12;
+        if (inb && c < cS) {
+            for (int L = 0; L < 3; L++) {
+                Sb[L] += (double)ex[L][a];
+                if (e2[0]) Sb2[L] += (double)e2[L][a];
+            }
+            nS += 1.0;
+        }
+        if (c < cend) {
+            if (c >= 1) {
+                if (!inb) firstoob = min(firstoob, c);
+                else if (C.lapd) { sL += (double)C.lapd[a]; nL += 1.0; }
+            }
+            if (C.mask && x >= lo1 && x < hi1 && y >= lo2 && y < hi2 && z >= lo3 && z < hi3)
+                C.mask[a] = 1;
+        }
+    }
+    for (int L = 0; L < 3; L++) { Sb[L] = X::sum(Sb[L]); Sb2[L] = X::sum(Sb2[L]); }
+    nS = X::sum(nS);
+    if constexpr (COOP) {
+        for (int o2 = 32; o2 >= 1; o2 >>= 1) firstoob = min(firstoob, __shfl_xor(firstoob, o2, 64));
+    }
+    double d2F = 0.0;
+    if (C.lapd) {
+        if (firstoob == 0x7fffffff) {
+            sL = X::sum(sL);
+            nL = X::sum(nL);
+        } else {
+            // stop at the first out-of-bounds cell: recount cells [1, firstoob)
+            sL = 0.0; nL = 0.0;
+            for (int c = 1 + X::lane(); c < firstoob; c += X::W) {
+                const int pc = C.cell[c];
+                const int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+                sL += (double)C.lapd[x + y * n1 + z * n12];
+                nL += 1.0;
+            }
+            sL = X::sum(sL);
+            nL = X::sum(nL);
+        }
+        if (nL > 0.0) d2F = sL / nL;
+    }
+    if (X::leader()) {
+        for (int L = 0; L < 3; L++) {
+            R.Sbar[o * 3 + L] = nS > 0.0 ? Sb[L] / nS : 0.0;
+            R.Sbar2[o * 3 + L] = nS > 0.0 ? Sb2[L] / nS : 0.0;
+        }
+        R.d2F[o] = d2F;
+    }
+}
+
+// The per-peak analysis after the gather.  P provides rad(s), Fbar(s),
+// count(s) and strain_inputs(s); with FINISH it also provides S / S2 / lap
+// sums per shell and computes Sbar, Sbar2, d2F and the mask here; without
+// FINISH those are left to finish_cells() and the final m0 is recorded.
+template <bool COOP, bool FINISH, class PV>
+__device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C, const ResultView &R, int64_t o)
+{
+    using X = Exec<COOP>;
+    const int lane = X::lane();
     const int m = pk.m;
     int status = DUMP_NONE;
     double RTHL = -1.0;
@@ -276,7 +385,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         if (mrf > 1) { int a = first_rad_gt(P, 1, mrf - 1, rlow_rf); if (a <= mrf - 1) mlow_rf = a; }
         int mupp_rf = mrf;
         { int b = last_rad_lt(P, mrf + 1, m, P.rad(mrf) + 2.0); if (b >= mrf + 1) mupp_rf = b; }
-        Strain srf = kernel_strain(P, C, mrf, mlow_rf, mupp_rf);
+        Strain srf = kernel_strain<COOP>(P, C, mrf, mlow_rf, mupp_rf);
         const double gradpkrf[3] = {srf.g[0], srf.g[1], srf.g[2]};
 
         // ---- Phase 4: fcrit crossing (RadialShell.jl:492-538) ----
@@ -284,11 +393,16 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         int m0 = pk.m0, mupp;
         if (P.Fbar(m0) >= fcrit) {
             int found = 0;
-            for (int base = m0; base <= m; base += 64) {
-                int mm = base + lane;
-                bool hit = mm <= m && P.Fbar(mm) < fcrit;
-                unsigned long long b = __ballot(hit);
-                if (b) { m0 = base + __ffsll((long long)b) - 1; found = 1; break; }
+            if constexpr (COOP) {
+                for (int base = m0; base <= m; base += 64) {
+                    int mm = base + lane;
+                    bool hit = mm <= m && P.Fbar(mm) < fcrit;
+                    unsigned long long b = __ballot(hit);
+                    if (b) { m0 = base + __ffsll((long long)b) - 1; found = 1; break; }
+                }
+            } else {
+                for (int mm = m0; mm <= m; mm++)
+                    if (P.Fbar(mm) < fcrit) { m0 = mm; found = 1; break; }
             }
             if (!found) mupp = m;
             else {
@@ -298,11 +412,18 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         } else {
             int mstart = m0 - 1 > 1 ? m0 - 1 : 1;
             int found = 0;
-            for (int top = mstart; top >= 1; top -= 64) {
-                int mp = top - lane;
-                bool hit = mp >= 1 && P.Fbar(mp) >= fcrit;
-                unsigned long long b = __ballot(hit);
-                if (b) { m0 = top - (__ffsll((long long)b) - 1) + 1; found = 1; break; }
+            if constexpr (COOP) {
+                for (int top = mstart; top >= 1; top -= 64) {
+                    int mp = top - lane;
+                    bool hit = mp >= 1 && P.Fbar(mp) >= fcrit;
+                    unsigned long long b = __ballot(hit);
+                    if (b) { m0 = top - (__ffsll((long long)b) - 1) + 1; found = 1; break; }
+                }
+            } else {
+                for (int mp = mstart; mp >= 1; mp--) {
+                    if (P.Fbar(mp) >= fcrit) { found = 1; break; }
+                    m0 = mp;
+                }
             }
             if (!found) { status = DUMP_NO_FCRIT; goto nocol; }
             int b = last_rad_lt(P, m0 + 1, m, P.rad(m0) + 2.0);
@@ -314,7 +435,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         int mlow = m0;
         if (m0 > 1) { int a = first_rad_gt(P, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
         const double ZZon = pk.ZZon;
-        Strain s0 = kernel_strain(P, C, m0, mlow, mupp);
+        Strain s0 = kernel_strain<COOP>(P, C, m0, mlow, mupp);
         if (!finite6(s0.E)) { status = DUMP_NONFINITE; goto nocol; }
         double Frho_m0, e_v_m0, p_v_m0;
         evals_ep(s0.E, Frho_m0, e_v_m0, p_v_m0);
@@ -325,15 +446,11 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         double Frhoh = Frho_m0, Frho = P.Fbar(m0), e_v = e_v_m0, p_v = p_v_m0;
         for (int i = 0; i < 6; i++) strain_mat[i] = s0.E[i];
         for (int L = 0; L < 3; L++) { gradpk_prev[L] = s0.g[L]; gradpkf_prev[L] = s0.gf[L]; }
-        // zvir1p at m0 is -1 (RadialShell.jl:581-583), never >= ZZon > 0 branch
-        // is kept for ZZon <= -1 inputs:
         bool collapsed = false;
+        // zvir1p at m0 is -1 (RadialShell.jl:581-583): collapsed at m0 only if ZZon <= -1
         if (-1.0 >= ZZon) {
-            zvir1 = -1.0;
-            zvir1p = -1.0;
             collapsed = true;
         } else {
-            zvir1 = -1.0;
             if (m0 == 1) { status = DUMP_M0_ONE; goto nocol; }
             for (int mp = m0 - 1; mp >= 1; mp--) {
                 double rupp_mp = P.rad(mp) + 2.0;
@@ -346,7 +463,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     int a = first_rad_gt(P, 1, mlow - 1, rlow_mp);
                     mlow = a <= mlow - 1 ? a : mlow;
                 }
-                Strain smp = kernel_strain(P, C, mp, mlow, mupp);
+                Strain smp = kernel_strain<COOP>(P, C, mp, mlow, mupp);
                 if (!finite6(smp.E)) { status = DUMP_NONFINITE; goto nocol; }
                 double Frho_mp, e_v_mp, p_v_mp;
                 evals_ep(smp.E, Frho_mp, e_v_mp, p_v_mp);
@@ -372,6 +489,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 last = smp;
             }
         }
+        (void)Frho;
         if (!collapsed) { status = DUMP_NO_COLLAPSE; goto nocol; }
         // only reachable with ZZon <= -1 (collapse flagged at m0 itself): the
         // reference would index rad[0]; report it as the m0 == 1 dump.
@@ -415,69 +533,75 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         const double sfac = Fbarx / Frhoh;
 
         // ---- Phase 7 (RadialShell.jl:756-819) ----
-        double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0, Srb = 0.0;
-        for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += 64) {
-            double cnt = P.count(m1);
-            if (cnt > 0.0) {
-                for (int L = 0; L < 3; L++) { Sb[L] += P.S(m1, L); Sb2[L] += P.S2(m1, L); }
-                nS += cnt;
-            }
-            if (m1 >= 2) {
-                double ra = P.rad(m1 - 1), rb = P.rad(m1);
-                double r5a = ra * ra * ra * ra * ra, r5b = rb * rb * rb * rb * rb;
-                if (m1 == 2) r5a = 0.0;
-                Srb += 0.5 * (P.Fbar(m1 - 1) + P.Fbar(m1)) * (r5b - r5a);
-            }
+        double Srb = 0.0;
+        for (int m1 = 2 + lane; m1 <= m0 - 1; m1 += X::W) {
+            double ra = P.rad(m1 - 1), rb = P.rad(m1);
+            double r5a = m1 == 2 ? 0.0 : ra * ra * ra * ra * ra, r5b = rb * rb * rb * rb * rb;
+            Srb += 0.5 * (P.Fbar(m1 - 1) + P.Fbar(m1)) * (r5b - r5a);
         }
-        for (int L = 0; L < 3; L++) { Sb[L] = wave_sum(Sb[L]); Sb2[L] = wave_sum(Sb2[L]); }
-        nS = wave_sum(nS);
-        Srb = wave_sum(Srb);
-        if (nS > 0.0) for (int L = 0; L < 3; L++) { Sb[L] /= nS; Sb2[L] /= nS; }
+        Srb = X::sum(Srb);
         if (zvir1 > 0.0 && dZvir != 0.0) {
             double rp = m0 - 1 >= 2 ? P.rad(m0 - 1) : 0.0;
             double rad5p = rp * rp * rp * rp * rp;
             Srb += 0.5 * (P.Fbar(m0 - 1) + Fbarx) * (RTHL5 - rad5p);
         }
         if (Fbarx * RTHL5 > 0.0) Srb /= (Fbarx * RTHL5);
-
         // cells with r <= RTHL: shells 1..q
         const int q = last_rad_lt(P, 1, m, nextafter(RTHL, 1e300));
-        const int cend = min(C.shell_start[q], pk.npart);  // cells [0, cend)
-        const int Rq = (int)floor(P.rad(q));
-        const bool inside_q = pk.ci - Rq >= 0 && pk.ci + Rq < C.n1 && pk.cj - Rq >= 0 && pk.cj + Rq < C.n2 &&
-                              pk.ck - Rq >= 0 && pk.ck + Rq < C.n3;
-        if (C.mask) {
-            const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
-            const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
-            for (int c = lane; c < cend; c += 64) {
-                int pc = C.cell[c];
-                int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
-                if (x < lo1 || x >= hi1 || y < lo2 || y >= hi2 || z < lo3 || z >= hi3) continue;
-                C.mask[x + (long)y * C.n1 + (long)z * C.n1 * C.n2] = 1;
+
+        if constexpr (FINISH) {
+            double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
+            for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += X::W) {
+                double cnt = P.count(m1);
+                if (cnt > 0.0) {
+                    for (int L = 0; L < 3; L++) { Sb[L] += P.S(m1, L); Sb2[L] += P.S2(m1, L); }
+                    nS += cnt;
+                }
             }
-        }
-        double d2F = 0.0;
-        if (P.has_lapd()) {
-            if (inside_q) {
-                double sL = 0.0, sn = 0.0;
-                for (int m1 = 2 + lane; m1 <= q; m1 += 64) { sL += P.lap(m1); sn += P.count(m1); }
-                sL = wave_sum(sL);
-                sn = wave_sum(sn);
-                if (sn > 0.0) d2F = sL / sn;
-            } else if (lane == 0) {
-                // RadialShell.jl:804-819: stop at the first out-of-bounds cell
-                double s = 0.0;
-                long nav = 0;
-                for (int c = 1; c < cend; c++) {
+            for (int L = 0; L < 3; L++) { Sb[L] = X::sum(Sb[L]); Sb2[L] = X::sum(Sb2[L]); }
+            nS = X::sum(nS);
+            if (nS > 0.0) for (int L = 0; L < 3; L++) { Sb[L] /= nS; Sb2[L] /= nS; }
+            const int cend = min(C.shell_start[q], pk.npart);
+            const int Rq = (int)floor(P.rad(q));
+            const bool inside_q = pk.ci - Rq >= 0 && pk.ci + Rq < C.n1 && pk.cj - Rq >= 0 && pk.cj + Rq < C.n2 &&
+                                  pk.ck - Rq >= 0 && pk.ck + Rq < C.n3;
+            if (C.mask) {
+                const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
+                const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
+                for (int c = lane; c < cend; c += X::W) {
                     int pc = C.cell[c];
                     int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
-                    if (x < 0 || x >= C.n1 || y < 0 || y >= C.n2 || z < 0 || z >= C.n3) break;
-                    nav++;
-                    s += (double)C.lapd[x + (long)y * C.n1 + (long)z * C.n1 * C.n2];
+                    if (x < lo1 || x >= hi1 || y < lo2 || y >= hi2 || z < lo3 || z >= hi3) continue;
+                    C.mask[x + (long)y * C.n1 + (long)z * C.n1 * C.n2] = 1;
                 }
-                d2F = nav > 0 ? s / (double)nav : 0.0;
             }
-            d2F = __shfl(d2F, 0, 64);
+            double d2F = 0.0;
+            if (P.has_lapd()) {
+                if (inside_q) {
+                    double sL = 0.0, sn = 0.0;
+                    for (int m1 = 2 + lane; m1 <= q; m1 += X::W) { sL += P.lap(m1); sn += P.count(m1); }
+                    sL = X::sum(sL);
+                    sn = X::sum(sn);
+                    if (sn > 0.0) d2F = sL / sn;
+                } else if (X::leader()) {
+                    // RadialShell.jl:804-819: stop at the first out-of-bounds cell
+                    double sacc = 0.0;
+                    long nav = 0;
+                    for (int c = 1; c < cend; c++) {
+                        int pc = C.cell[c];
+                        int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+                        if (x < 0 || x >= C.n1 || y < 0 || y >= C.n2 || z < 0 || z >= C.n3) break;
+                        nav++;
+                        sacc += (double)C.lapd[x + (long)y * C.n1 + (long)z * C.n1 * C.n2];
+                    }
+                    d2F = nav > 0 ? sacc / (double)nav : 0.0;
+                }
+                if constexpr (COOP) d2F = __shfl(d2F, 0, 64);
+            }
+            if (X::leader()) {
+                for (int L = 0; L < 3; L++) { R.Sbar[o * 3 + L] = Sb[L]; R.Sbar2[o * 3 + L] = Sb2[L]; }
+                R.d2F[o] = d2F;
+            }
         }
 
         // ---- zvir_half (RadialShell.jl:821-885) ----
@@ -486,7 +610,6 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
             for (int jj = 1; jj <= 10; jj++) {
                 double rcur = RTHL * pow((double)(jj - 1) / 9.0 * 0.5 + 0.5, 1.0 / 3.0);
                 double rupp_rc = rcur + 2.0;
-                // first m1 in 2..m-1 with rad > rupp_rc, else m-1 (1 if m <= 2)
                 int mupp_rc = 1;
                 if (m - 1 >= 2) {
                     int f = first_rad_gt(P, 2, m - 1, rupp_rc);
@@ -500,7 +623,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     int f = first_rad_gt(P, 1, m - 1, rcur);
                     m0_rc = f <= m - 1 ? f : m - 1;
                 }
-                Strain src = kernel_strain(P, C, m0_rc, mlow_rc, mupp_rc);
+                Strain src = kernel_strain<COOP>(P, C, m0_rc, mlow_rc, mupp_rc);
                 if (!finite6(src.E)) { status = DUMP_NONFINITE; goto nocol; }
                 double Frho_rc, e_rc, p_rc;
                 evals_ep(src.E, Frho_rc, e_rc, p_rc);
@@ -512,37 +635,33 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
             }
         }
 
-        if (lane == 0) {
-            res[R_RTHL] = RTHL;
-            res[R_SRB] = Srb;
+        if (X::leader()) {
+            R.RTHL[o] = RTHL;
+            R.Srb[o] = Srb;
             for (int L = 0; L < 3; L++) {
-                res[R_SBAR + L] = Sb[L];
-                res[R_SBAR2 + L] = Sb2[L];
-                res[R_GRADPK + L] = gpk[L];
-                res[R_GRADPKRF + L] = gradpkrf[L];
-                res[R_GRADPKF + L] = gpkf[L];
+                R.gradpk[o * 3 + L] = gpk[L];
+                R.gradpkrf[o * 3 + L] = gradpkrf[L];
+                R.gradpkf[o * 3 + L] = gpkf[L];
             }
             // strain_mat = Ebar_last * (Fbarx / Frhoh), column-major 3x3
             const double *E = last.E;
-            res[R_STRAIN + 0] = E[0] * sfac; res[R_STRAIN + 4] = E[1] * sfac; res[R_STRAIN + 8] = E[2] * sfac;
-            res[R_STRAIN + 1] = res[R_STRAIN + 3] = E[3] * sfac;
-            res[R_STRAIN + 2] = res[R_STRAIN + 6] = E[4] * sfac;
-            res[R_STRAIN + 5] = res[R_STRAIN + 7] = E[5] * sfac;
-            res[R_FBARX] = Fbarx;
-            res[R_EV] = e_v_f;
-            res[R_PV] = p_v_f;
-            res[R_ZVIRH] = zvir_half;
-            res[R_D2F] = d2F;
-            res[R_NREC - 1] = 0.0;
+            double *S = R.strain + o * 9;
+            S[0] = E[0] * sfac; S[4] = E[1] * sfac; S[8] = E[2] * sfac;
+            S[1] = S[3] = E[3] * sfac;
+            S[2] = S[6] = E[4] * sfac;
+            S[5] = S[7] = E[5] * sfac;
+            R.Fbarx[o] = Fbarx;
+            R.e_v[o] = e_v_f;
+            R.p_v[o] = p_v_f;
+            R.zvir_half[o] = zvir_half;
+            R.status[o] = DUMP_NONE;
+            R.m0[o] = m0;
+            R.q[o] = q;
         }
         return DUMP_NONE;
     }
 nocol:
-    if (lane == 0) {
-        for (int i = 0; i < R_NREC; i++) res[i] = 0.0;
-        res[R_RTHL] = -1.0;
-        res[R_ZVIRH] = -1.0;
-    }
+    if (X::leader()) write_nocollapse(R, o, status);
     return status;
 }
 

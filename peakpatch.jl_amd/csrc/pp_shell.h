@@ -9,20 +9,41 @@
 namespace pp {
 
 // Device pointers of the PeakGrid fields (RadialShell.jl:45-56); eta2 / lapd
-// may be NULL.
+// may be NULL.  dT is the library's transposed copy of delta (dT(x,y,z) =
+// delta(z,x,y)), made by the persistent form.
 struct GatherFields {
     const float *d, *ex, *ey, *ez, *e2x, *e2y, *e2z, *lap;
+    const float *dT;
     int n;
+};
+
+struct BatchDev {
+    const int32_t *i, *j, *k;
+    const double *Rf;
+    const int64_t *ir2min;
+    const double *ZZon, *fcrit;
+};
+
+// One pp_analyse_peaks call: device batch, per-peak params, results.
+struct ShellRun {
+    int64_t np = 0;
+    BatchDev batch;
+    PeakParam *pp = nullptr;   // params in the caller's order
+    int32_t *order = nullptr;  // processing order (indices into pp)
+    int32_t *keys = nullptr;   // scratch for the sort
+    ResultView R;
 };
 
 int check_fields(pp_ctx *ctx, const pp_fields *f);
 int upload_fields(pp_ctx *ctx, const pp_fields *f, GatherFields &G);
-int build_peak_params(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, double rmax2rs,
-                      const int8_t *mask_host, std::vector<PeakParam> &out);
 int prepare_post_const(pp_ctx *ctx, int n, int nbuff, double alatt, const float *lapd, int8_t *mask,
                        PostConst &C);
-int run_persistent(pp_ctx *ctx, const GatherFields &G, std::vector<PeakParam> &pp, PostConst &C,
-                   double *d_res, int32_t *d_st, PeakParam *d_pp);
+int prepare_batch(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, double rmax2rs,
+                  const int8_t *d_mask, ShellRun &S);
+int run_plain(pp_ctx *ctx, const GatherFields &G, ShellRun &S, PostConst &C, std::vector<int64_t> *offs_out);
+int run_persistent(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C);
+int persistent_profiles(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, std::vector<int32_t> &cnt,
+                        std::vector<double> &fbar, std::vector<int64_t> &off);
 int autotune_persistent(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks, double alatt,
                         double rmax2rs, int *choice);
 

@@ -1,17 +1,20 @@
 // pp_shell.hip -- per-peak radial-shell analysis (analyse_peak,
-// src/HaloFinder/RadialShell.jl:282-891) on the GPU.
+// src/HaloFinder/RadialShell.jl:282-891) on the GPU: host orchestration,
+// device-side per-peak parameters, and the plain kernel form.
 //
-// Two kernel forms share the post-process in pp_post.h:
+// Kernel forms (both end in the post-process of pp_post.h):
 //  * PP_SHELL_PLAIN: one 64-lane workgroup per peak, one launch per batch.
 //    The wave walks the r^2-sorted cell table shell by shell, reduces each
-//    shell with a wave butterfly and keeps the profile in global memory
-//    (the structure of the reference's _shell_gather_full_kernel! +
-//    _post_process_kernel!, ext/CUDAExt.jl:859-2051, re-done for wave64).
-//  * PP_SHELL_PERSISTENT: see pp_persist.hip.
+//    shell with a wave butterfly and keeps the profile in global memory (the
+//    structure of the reference's _shell_gather_full_kernel! +
+//    _post_process_kernel!, ext/CUDAExt.jl:859-2051, done for wave64).
+//  * PP_SHELL_PERSISTENT: pp_persist.hip.
 //
 // Semantics follow the production CPU code, not the reference's GPU
 // prototype: npart cell cut for rmax2rs > 0, d2F from cell 2 with the stop
 // at the first out-of-bounds cell, zvir_half computed.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -21,6 +24,89 @@
 #include "pp_shell.h"
 
 namespace pp {
+
+// ------------------------------------------------------ per-peak params
+__device__ __forceinline__ int d_shell_of_cell(const int32_t *start, int nshells, int c)
+{
+    // 1-based shell s with start[s-1] <= c < start[s] (upper_bound over start[0..nshells])
+    int lo = 0, hi = nshells + 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= c) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// npart (RadialShell.jl:289-295), finalised shells m (the last shell touched
+// by cells 2..npart is never finalised, :379-453), m0 (first finalised shell
+// with r^2 > ir2min, :392-399), the Phase-3 shell mrf (:457-471: only the first
+// cell of an r^2 can improve the distance and it sees the counter before its
+// increment, so slot s maps to s-2, slot 2 to 1), the in-grid flag and the
+// rmax2rs pre-mask (:311-316).
+__global__ void k_peak_params(BatchDev b, int64_t np, const int32_t *__restrict__ start,
+                              const int32_t *__restrict__ r2, int nshells, int64_t ncells, int n, double alatt,
+                              double rmax2rs, double f3p, const int8_t *__restrict__ mask, PeakParam *__restrict__ out,
+                              int *__restrict__ nbad)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        PeakParam q;
+        const int i = b.i[p], j = b.j[p], k = b.k[p];
+        q.ci = i; q.cj = j; q.ck = k;
+        q.order = p;
+        q.ZZon = b.ZZon[p];
+        q.fcrit = b.fcrit[p];
+        const bool valid = i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n;
+        if (!valid) {
+            atomicAdd(nbad, 1);
+            q.ci = q.cj = q.ck = 0;
+        }
+        const double Rf = b.Rf[p];
+        int64_t npart = ncells;
+        if (rmax2rs > 0.0) {
+            const double t = rmax2rs * Rf / alatt;
+            const double v = floor(f3p * (t * t * t));
+            if (v < (double)npart) npart = v < 0.0 ? 0 : (int64_t)v;
+        }
+        q.npart = (int32_t)npart;
+        int slast = npart >= 2 ? d_shell_of_cell(start, nshells, (int)(npart - 1)) : 1;
+        q.m = npart >= 2 ? slast - 1 : 1;
+        // mrf: argmin over slots 2..slast of |r2 - tgt| (first on ties), if < 10
+        const double tgt = (Rf / alatt) * (Rf / alatt);
+        int mrf = 1;
+        if (slast >= 2) {
+            int lo = 2, hi = slast + 1;  // first s in [2, slast] with r2 >= tgt
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if ((double)r2[mid - 1] >= tgt) hi = mid; else lo = mid + 1;
+            }
+            int best = -1;
+            double bd = 10.0;
+            if (lo - 1 >= 2) { double d = fabs((double)r2[lo - 2] - tgt); if (d < bd) { bd = d; best = lo - 1; } }
+            if (lo <= slast) { double d = fabs((double)r2[lo - 1] - tgt); if (d < bd) { bd = d; best = lo; } }
+            if (best >= 2) mrf = best == 2 ? 1 : best - 2;
+        }
+        q.mrf = mrf;
+        int m0 = q.m >= 2 ? q.m : 1;
+        if (q.m >= 2) {
+            const int64_t thr = b.ir2min[p];
+            int lo = 2, hi = q.m + 1;  // first s in [2, m] with r2 > ir2min
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if ((int64_t)r2[mid - 1] > thr) hi = mid; else lo = mid + 1;
+            }
+            if (lo <= q.m) m0 = lo;
+        }
+        q.m0 = m0;
+        const int R = (int)floor(sqrt((double)r2[q.m - 1]));
+        const bool inside = valid && i - R >= 0 && i + R < n && j - R >= 0 && j + R < n && k - R >= 0 && k + R < n;
+        q.flags = (inside ? 1 : 0) | (valid ? 0 : 8);
+        if (valid && rmax2rs > 0.0 && mask) {
+            const double lim = floor(Rf / alatt * rmax2rs);
+            if ((double)mask[i + (int64_t)j * n + (int64_t)k * n * n] > lim) q.flags |= 2;
+        }
+        out[p] = q;
+    }
+}
 
 // --------------------------------------------------------- profile views
 // Plain form: profile in global memory, SoA [field][shell] per peak.
@@ -37,11 +123,11 @@ struct GlobalProfile {
     __device__ double S2(int s, int L) const { return (double)f[(PF_S2 + L) * m + s - 1]; }
     __device__ double lap(int s) const { return (double)f[PF_L * m + s - 1]; }
     __device__ bool has_lapd() const { return lapd; }
-    __device__ void strain_inputs(int s, double G[3], double Gf[3], double SR[9]) const
+    __device__ void strain_inputs(int s, double G[3], double Gf[3], double Es[6]) const
     {
         if (s == 1) {
             for (int i = 0; i < 3; i++) G[i] = Gf[i] = 0.0;
-            for (int i = 0; i < 9; i++) SR[i] = 0.0;
+            for (int i = 0; i < 6; i++) Es[i] = 0.0;
             return;
         }
         const double r = radt[s - 1];
@@ -49,7 +135,10 @@ struct GlobalProfile {
             G[i] = (double)f[(PF_G + i) * m + s - 1] / r;
             Gf[i] = (double)f[(PF_GF + i) * m + s - 1] / r;
         }
+        double SR[9];
         for (int i = 0; i < 9; i++) SR[i] = (double)f[(PF_SR + i) * m + s - 1] / r;
+        Es[0] = SR[0] + SR[0]; Es[1] = SR[4] + SR[4]; Es[2] = SR[8] + SR[8];
+        Es[3] = SR[1] + SR[3]; Es[4] = SR[2] + SR[6]; Es[5] = SR[5] + SR[7];
     }
 };
 
@@ -99,10 +188,9 @@ __device__ void fbar_recursion(const float *f, int m, const double *radt, double
 
 __global__ void __launch_bounds__(64) k_shell_plain(GatherFields F, const PeakParam *__restrict__ pks,
                                                     const int64_t *__restrict__ prof_off, float *__restrict__ prof,
-                                                    double *__restrict__ fbar, PostConst C, double *__restrict__ res,
-                                                    int32_t *__restrict__ status, int64_t p0)
+                                                    double *__restrict__ fbar, PostConst C, ResultView R)
 {
-    const int64_t p = p0 + blockIdx.x;
+    const int64_t p = blockIdx.x;
     const int lane = threadIdx.x;
     const PeakParam pk = pks[p];
     const int m = pk.m;
@@ -145,8 +233,31 @@ __global__ void __launch_bounds__(64) k_shell_plain(GatherFields F, const PeakPa
     __threadfence_block();
     __syncthreads();
     GlobalProfile P{pf, fb, C.rad, m, F.lap != nullptr};
-    int st = analyse_post(P, pk, C, res + p * R_NREC);
-    if (lane == 0) status[p] = st;
+    analyse_post<true, true>(P, pk, C, R, pk.order);
+    if (lane == 0) R.nshells[pk.order] = m;
+}
+
+__global__ void k_status_hist(const int32_t *__restrict__ st, int64_t np, unsigned long long *__restrict__ hist)
+{
+    __shared__ unsigned int h[8];
+    if (threadIdx.x < 8) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int s = st[p];
+        if (s >= 0 && s < 8) atomicAdd(&h[s], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+__global__ void k_fill_i32(int *p, int n, int v)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
 // ------------------------------------------------------------- host side
@@ -199,83 +310,6 @@ int prepare_post_const(pp_ctx *ctx, int n, int nbuff, double alatt, const float 
     return PP_OK;
 }
 
-// Per-peak parameters (host, Float64): npart, finalised shells m, the
-// initial m0 and the Phase-3 shell mrf.
-int build_peak_params(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, double rmax2rs,
-                      const int8_t *mask_host, std::vector<PeakParam> &out)
-{
-    const ShellTable &T = ctx->shells;
-    const int64_t np = pb->npeaks;
-    out.resize(np);
-    // shell (1-based) of a cell index (0-based)
-    auto shell_of_cell = [&](int64_t c) {
-        return (int)(std::upper_bound(T.shell_start.begin(), T.shell_start.end(), (int32_t)c) -
-                     T.shell_start.begin());
-    };
-    struct RfCache { double Rf; int npart, m, mrf; };
-    std::vector<RfCache> cache;
-    for (int64_t p = 0; p < np; p++) {
-        PeakParam &q = out[p];
-        const int i = pb->i[p], j = pb->j[p], k = pb->k[p];
-        PP_ARG(i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n, "peak %lld at (%d,%d,%d) outside the %d^3 grid",
-               (long long)p, i, j, k, n);
-        const double Rf = pb->Rf[p];
-        const RfCache *rc = nullptr;
-        for (const auto &e : cache) if (e.Rf == Rf) { rc = &e; break; }
-        if (!rc) {
-            RfCache e;
-            e.Rf = Rf;
-            int64_t npart = T.ncells;
-            if (rmax2rs > 0.0) {
-                const double f3p = 4.0 / 3.0 * M_PI;
-                const double t = rmax2rs * Rf / alatt;
-                double v = std::floor(f3p * (t * t * t));
-                if (v < (double)npart) npart = v < 0 ? 0 : (int64_t)v;
-            }
-            e.npart = (int)npart;
-            e.m = npart >= 2 ? shell_of_cell(npart - 1) - 1 : 1;
-            // Phase 3 (RadialShell.jl:457-471): only the first cell of each
-            // r^2 can improve the distance; it sees the shell counter before
-            // its increment (slot - 2; slot 2 sees 1).
-            const double tgt = (Rf / alatt) * (Rf / alatt);
-            double rmrf = 10.0;
-            int mrf = 1;
-            if (npart >= 2) {
-                const int slast = shell_of_cell(npart - 1);
-                for (int s = 2; s <= slast; s++) {
-                    double dd = std::fabs((double)T.shell_r2[s - 1] - tgt);
-                    if (dd < rmrf) { mrf = s == 2 ? 1 : s - 2; rmrf = dd; }
-                }
-            }
-            e.mrf = mrf;
-            cache.push_back(e);
-            rc = &cache.back();
-        }
-        q.ci = i; q.cj = j; q.ck = k;
-        q.m = rc->m;
-        q.mrf = rc->mrf;
-        q.npart = rc->npart;
-        // m0 (RadialShell.jl:392-399): first finalised shell with r^2 > ir2min
-        int m0 = q.m >= 2 ? q.m : 1;
-        for (int s = 2; s <= q.m; s++)
-            if ((int64_t)T.shell_r2[s - 1] > pb->ir2min[p]) { m0 = s; break; }
-        q.m0 = m0;
-        const int R = (int)std::floor(std::sqrt((double)T.shell_r2[q.m - 1]));
-        const bool inside = i - R >= 0 && i + R < n && j - R >= 0 && j + R < n && k - R >= 0 && k + R < n;
-        q.flags = inside ? 1 : 0;
-        // rmax2rs pre-mask (RadialShell.jl:311-316), read from the mask as it
-        // is when the batch starts.
-        if (rmax2rs > 0.0 && mask_host) {
-            double lim = std::floor(Rf / alatt * rmax2rs);
-            if ((double)mask_host[i + (int64_t)j * n + (int64_t)k * n * n] > lim) q.flags |= 2;
-        }
-        q.ZZon = pb->ZZon[p];
-        q.fcrit = pb->fcrit[p];
-        q.order = p;
-    }
-    return PP_OK;
-}
-
 int upload_fields(pp_ctx *ctx, const pp_fields *f, GatherFields &G)
 {
     const int n = f->n1;
@@ -291,6 +325,7 @@ int upload_fields(pp_ctx *ctx, const pp_fields *f, GatherFields &G)
     }
     G.d = dst[0]; G.ex = dst[1]; G.ey = dst[2]; G.ez = dst[3];
     G.e2x = dst[4]; G.e2y = dst[5]; G.e2z = dst[6]; G.lap = dst[7];
+    G.dT = nullptr;
     G.n = n;
     return PP_OK;
 }
@@ -300,6 +335,7 @@ int check_fields(pp_ctx *ctx, const pp_fields *f)
     PP_ARG(f && f->delta && f->etax && f->etay && f->etaz, "fields: delta and eta x/y/z are required");
     PP_ARG(f->n1 == f->n2 && f->n2 == f->n3 && f->n1 >= 3,
            "fields must be a cubic grid (the transposed delta read of RadialShell.jl:448 assumes n1 = n2 = n3)");
+    PP_ARG(f->n1 <= 1290, "grid %d too large for 32-bit cell offsets", f->n1);
     PP_ARG((f->eta2x == nullptr) == (f->eta2y == nullptr) && (f->eta2y == nullptr) == (f->eta2z == nullptr),
            "eta2 x/y/z must be all given or all NULL");
     if (!ctx->have_table) { set_error("collapse table not uploaded"); return PP_ERR_STATE; }
@@ -307,67 +343,131 @@ int check_fields(pp_ctx *ctx, const pp_fields *f)
     return PP_OK;
 }
 
-static void unpack_results(const std::vector<double> &R, const std::vector<int32_t> &st,
-                           const std::vector<PeakParam> &pp, pp_results *out)
+// Device copy of the batch + per-peak parameters.  Fills `S` (pointers into
+// ctx scratch) and returns PP_ERR_ARG if any peak lies outside the grid.
+int prepare_batch(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, double rmax2rs,
+                  const int8_t *d_mask, ShellRun &S)
 {
-    const int64_t np = (int64_t)pp.size();
-    for (int q = 0; q < 6; q++) out->dump_counts[q] = 0;
-    for (int64_t p = 0; p < np; p++) {
-        const double *r = &R[(size_t)p * R_NREC];
-        const int64_t o = pp[p].order;
-        if (out->RTHL) out->RTHL[o] = r[R_RTHL];
-        if (out->Srb) out->Srb[o] = r[R_SRB];
-        for (int L = 0; L < 3; L++) {
-            if (out->Sbar) out->Sbar[o * 3 + L] = r[R_SBAR + L];
-            if (out->Sbar2) out->Sbar2[o * 3 + L] = r[R_SBAR2 + L];
-            if (out->gradpk) out->gradpk[o * 3 + L] = r[R_GRADPK + L];
-            if (out->gradpkrf) out->gradpkrf[o * 3 + L] = r[R_GRADPKRF + L];
-            if (out->gradpkf) out->gradpkf[o * 3 + L] = r[R_GRADPKF + L];
+    const int64_t np = pb->npeaks;
+    const ShellTable &T = ctx->shells;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bb = al(np * 4) * 3 + al(np * 8) * 4;          // i,j,k + Rf, ir2min, ZZon, fcrit
+    const size_t pb_ = al(np * sizeof(PeakParam));
+    const size_t ob = al(np * 4) * 2;                           // order + sort keys
+    const size_t rb = al(np * 8) * 31 + al(np * 4) * 4;         // 31 doubles + status, nshells, m0, q
+    PP_CHECK(ctx->d_pk.grow(bb + pb_ + ob + rb + 1024));
+    char *p = ctx->d_pk.as<char>();
+    BatchDev B;
+    if (pb->on_device) {
+        B.i = pb->i; B.j = pb->j; B.k = pb->k; B.Rf = pb->Rf; B.ir2min = pb->ir2min; B.ZZon = pb->ZZon;
+        B.fcrit = pb->fcrit;
+    } else {
+        const void *src[7] = {pb->i, pb->j, pb->k, pb->Rf, pb->ir2min, pb->ZZon, pb->fcrit};
+        const size_t sz[7] = {4, 4, 4, 8, 8, 8, 8};
+        void *dst[7];
+        char *q = p;
+        for (int t = 0; t < 7; t++) {
+            dst[t] = q;
+            PP_HIP(hipMemcpyAsync(q, src[t], np * sz[t], hipMemcpyHostToDevice, ctx->stream));
+            q += al(np * sz[t]);
         }
-        if (out->strain) for (int q = 0; q < 9; q++) out->strain[o * 9 + q] = r[R_STRAIN + q];
-        if (out->Fbarx) out->Fbarx[o] = r[R_FBARX];
-        if (out->e_v) out->e_v[o] = r[R_EV];
-        if (out->p_v) out->p_v[o] = r[R_PV];
-        if (out->zvir_half) out->zvir_half[o] = r[R_ZVIRH];
-        if (out->d2F) out->d2F[o] = r[R_D2F];
-        if (out->status) out->status[o] = st[p];
-        if (out->nshells) out->nshells[o] = pp[p].m;
-        int s = st[p];
-        if (s >= 1 && s <= 5) out->dump_counts[s]++;
-        else if (s == 6) out->dump_counts[0]++;  // non-finite strain: counted apart
+        B.i = (const int32_t *)dst[0]; B.j = (const int32_t *)dst[1]; B.k = (const int32_t *)dst[2];
+        B.Rf = (const double *)dst[3]; B.ir2min = (const int64_t *)dst[4]; B.ZZon = (const double *)dst[5];
+        B.fcrit = (const double *)dst[6];
     }
+    p += bb;
+    S.batch = B;
+    S.pp = (PeakParam *)p; p += pb_;
+    S.order = (int32_t *)p; p += al(np * 4);
+    S.keys = (int32_t *)p; p += al(np * 4);
+    ResultView &R = S.R;
+    double **dptr[13] = {&R.RTHL, &R.Srb, &R.Sbar, &R.Sbar2, &R.strain, &R.gradpk, &R.gradpkrf, &R.Fbarx,
+                         &R.e_v, &R.p_v, &R.zvir_half, &R.gradpkf, &R.d2F};
+    const int width[13] = {1, 1, 3, 3, 9, 3, 3, 1, 1, 1, 1, 3, 1};
+    for (int t = 0; t < 13; t++) { *dptr[t] = (double *)p; p += al(np * 8 * width[t]); }
+    R.status = (int32_t *)p; p += al(np * 4);
+    R.nshells = (int32_t *)p; p += al(np * 4);
+    R.m0 = (int32_t *)p; p += al(np * 4);
+    R.q = (int32_t *)p; p += al(np * 4);
+    PP_CHECK(ctx->d_misc.grow(4096));
+    int *d_bad = ctx->d_misc.as<int>();
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, d_bad, 1, 0);
+    const int grid = (int)std::min<int64_t>((np + 255) / 256, (int64_t)ctx->num_cu * 8);
+    hipLaunchKernelGGL(k_peak_params, dim3(std::max(grid, 1)), dim3(256), 0, ctx->stream, B, np,
+                       T.d_shell_start.as<int32_t>(), T.d_shell_r2.as<int32_t>(), T.nshells, T.ncells, n, alatt,
+                       rmax2rs, 4.0 / 3.0 * M_PI, d_mask, S.pp, d_bad);
+    PP_HIP(hipGetLastError());
+    int bad = 0;
+    PP_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    PP_ARG(bad == 0, "%d peak(s) outside the %d^3 grid", bad, n);
+    S.np = np;
+    return PP_OK;
 }
 
-// Runs the plain kernel over params already on the device.
-static int run_plain(pp_ctx *ctx, const GatherFields &G, const std::vector<PeakParam> &pp,
-                     const PeakParam *d_pp, PostConst &C, double *d_res, int32_t *d_st)
+// Runs the plain kernel: one launch per chunk of peaks whose profiles fit the
+// scratch budget.
+int run_plain(pp_ctx *ctx, const GatherFields &G, ShellRun &S, PostConst &C, std::vector<int64_t> *offs_out)
 {
-    const int64_t np = (int64_t)pp.size();
-    // profile scratch, processed in chunks of bounded size
-    const size_t budget = (size_t)1 << 31;
+    const int64_t np = S.np;
+    std::vector<PeakParam> hp(np);
+    PP_HIP(hipMemcpyAsync(hp.data(), S.pp, np * sizeof(PeakParam), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    const size_t rec = PF_NFIELD * 4 + 8;
+    const int64_t budget = (int64_t)1 << 32;
     std::vector<int64_t> off(np);
     int64_t p = 0;
     while (p < np) {
         int64_t q = p, tot = 0;
-        while (q < np && (tot + pp[q].m) * (int64_t)(PF_NFIELD * 4 + 8) <= (int64_t)budget) {
+        while (q < np && (tot + hp[q].m) * (int64_t)rec <= budget) {
             off[q] = tot;
-            tot += pp[q].m;
+            tot += hp[q].m;
             q++;
         }
         if (q == p) { set_error("peak profile exceeds scratch budget"); return PP_ERR_OOM; }
-        PP_CHECK(ctx->d_prof.grow((size_t)tot * (PF_NFIELD * 4 + 8) + (size_t)(q - p) * 8 + 256));
+        if (offs_out && (p != 0 || q < np)) { set_error("profile export needs a single chunk"); return PP_ERR_ARG; }
+        const size_t offb = (((size_t)(q - p) * 8) + 255) & ~(size_t)255;
+        const size_t fb = (((size_t)tot * PF_NFIELD * 4) + 255) & ~(size_t)255;
+        PP_CHECK(ctx->d_prof.grow(offb + fb + (size_t)tot * 8 + 256));
         char *base = ctx->d_prof.as<char>();
         int64_t *d_off = (int64_t *)base;
-        float *d_f = (float *)(base + (((size_t)(q - p) * 8 + 255) & ~(size_t)255));
-        double *d_fb = (double *)((char *)d_f + (((size_t)tot * PF_NFIELD * 4 + 255) & ~(size_t)255));
+        float *d_f = (float *)(base + offb);
+        double *d_fb = (double *)(base + offb + fb);
         PP_HIP(hipMemcpyAsync(d_off, off.data() + p, (q - p) * 8, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_shell_plain, dim3((unsigned)(q - p)), dim3(64), 0, ctx->stream, G, d_pp + p,
-                           (const int64_t *)d_off, d_f, d_fb, C, d_res + p * R_NREC, d_st + p, (int64_t)0);
+        hipLaunchKernelGGL(k_shell_plain, dim3((unsigned)(q - p)), dim3(64), 0, ctx->stream, G, S.pp + p,
+                           (const int64_t *)d_off, d_f, d_fb, C, S.R);
         PP_HIP(hipGetLastError());
-        // the offsets buffer is reused by the next chunk: wait for this one
+        // the offsets buffer is reused by the next chunk
         PP_HIP(hipStreamSynchronize(ctx->stream));
         p = q;
     }
+    if (offs_out) *offs_out = off;
+    return PP_OK;
+}
+
+static int copy_results(pp_ctx *ctx, const ShellRun &S, pp_results *out)
+{
+    const int64_t np = S.np;
+    const hipMemcpyKind kind = out->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    const ResultView &R = S.R;
+    struct Cp { void *dst; const void *src; size_t b; };
+    const Cp cp[] = {
+        {out->RTHL, R.RTHL, 8}, {out->Srb, R.Srb, 8}, {out->Sbar, R.Sbar, 24}, {out->Sbar2, R.Sbar2, 24},
+        {out->strain, R.strain, 72}, {out->gradpk, R.gradpk, 24}, {out->gradpkrf, R.gradpkrf, 24},
+        {out->Fbarx, R.Fbarx, 8}, {out->e_v, R.e_v, 8}, {out->p_v, R.p_v, 8}, {out->zvir_half, R.zvir_half, 8},
+        {out->gradpkf, R.gradpkf, 24}, {out->d2F, R.d2F, 8}, {out->status, R.status, 4},
+        {out->nshells, R.nshells, 4}};
+    for (const Cp &c : cp)
+        if (c.dst) PP_HIP(hipMemcpyAsync(c.dst, c.src, c.b * np, kind, ctx->stream));
+    unsigned long long *d_h = (unsigned long long *)(ctx->d_misc.as<char>() + 512);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, d_h, 8, 0ull);
+    hipLaunchKernelGGL(k_status_hist, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((np + 255) / 256, 1024))),
+                       dim3(256), 0, ctx->stream, (const int32_t *)R.status, np, d_h);
+    PP_HIP(hipGetLastError());
+    unsigned long long h[8];
+    PP_HIP(hipMemcpyAsync(h, d_h, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < 7; q++) out->dump_counts[q] = (int64_t)h[q];
     return PP_OK;
 }
 
@@ -383,38 +483,21 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
 {
     PP_ARG(ctx && fields && peaks && out, "NULL argument");
     PP_ARG(peaks->npeaks >= 0, "negative npeaks");
+    PP_ARG(peaks->npeaks < (1ll << 31), "batch larger than 2^31 peaks");
     PP_ARG(alatt > 0.0, "alatt must be > 0");
     PP_ARG(kernel_form == PP_SHELL_PLAIN || kernel_form == PP_SHELL_PERSISTENT, "unknown kernel form %d", kernel_form);
     PP_CHECK(check_fields(ctx, fields));
     PP_CHECK(ensure_device(ctx));
-    for (int q = 0; q < 6; q++) out->dump_counts[q] = 0;
+    for (int q = 0; q < 7; q++) out->dump_counts[q] = 0;
     const int64_t np = peaks->npeaks;
     if (np == 0) return PP_OK;
     PP_ARG(peaks->i && peaks->j && peaks->k && peaks->Rf && peaks->ir2min && peaks->ZZon && peaks->fcrit,
            "peak batch arrays must all be given");
     const int n = fields->n1;
-    std::vector<int8_t> mask_host_copy;
-    const int8_t *mask_for_params = nullptr;
-    // the pre-mask test can only fire where floor(Rf / alatt * rmax2rs) == 0
-    bool premask_possible = false;
-    if (mask && rmax2rs > 0.0)
-        for (int64_t p = 0; p < np && !premask_possible; p++)
-            premask_possible = std::floor(peaks->Rf[p] / alatt * rmax2rs) < 1.0;
-    if (premask_possible) {
-        if (mask_on_device) {
-            mask_host_copy.resize((size_t)n * n * n);
-            PP_HIP(hipMemcpy(mask_host_copy.data(), mask, mask_host_copy.size(), hipMemcpyDeviceToHost));
-            mask_for_params = mask_host_copy.data();
-        } else {
-            mask_for_params = mask;
-        }
-    }
-    std::vector<PeakParam> pp;
-    PP_CHECK(build_peak_params(ctx, peaks, n, alatt, rmax2rs, mask_for_params, pp));
+    const size_t N = (size_t)n * n * n;
     GatherFields G;
     PP_CHECK(upload_fields(ctx, fields, G));
     int8_t *d_mask = nullptr;
-    const size_t N = (size_t)n * n * n;
     if (mask) {
         if (mask_on_device) d_mask = mask;
         else {
@@ -425,35 +508,25 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     }
     PostConst C;
     PP_CHECK(prepare_post_const(ctx, n, nbuff, alatt, G.lap, d_mask, C));
-    const size_t resb = (size_t)np * R_NREC * sizeof(double), stb = (size_t)np * sizeof(int32_t);
-    const size_t ppb = (size_t)np * sizeof(PeakParam);
-    PP_CHECK(ctx->d_res.grow(resb + stb + ppb + 1024));
-    double *d_res = ctx->d_res.as<double>();
-    int32_t *d_st = (int32_t *)((char *)d_res + resb);
-    PeakParam *d_pp = (PeakParam *)((char *)d_st + ((stb + 255) & ~(size_t)255));
-    int rc = PP_OK;
+    PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    ShellRun S;
+    PP_CHECK(prepare_batch(ctx, peaks, n, alatt, rmax2rs, d_mask, S));
     if (kernel_form == PP_SHELL_PERSISTENT) {
-        PP_CHECK(run_persistent(ctx, G, pp, C, d_res, d_st, d_pp));
+        PP_CHECK(run_persistent(ctx, G, S, C));
     } else {
-        PP_HIP(hipMemcpyAsync(d_pp, pp.data(), ppb, hipMemcpyHostToDevice, ctx->stream));
-        PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-        rc = run_plain(ctx, G, pp, d_pp, C, d_res, d_st);
-        PP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-        PP_CHECK(rc);
-        PP_HIP(hipStreamSynchronize(ctx->stream));
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-        ctx->timing.shell_ms = ms;
-        ctx->timing.gather_ms = ms;
+        PP_CHECK(run_plain(ctx, G, S, C, nullptr));
     }
-    std::vector<double> R((size_t)np * R_NREC);
-    std::vector<int32_t> st(np);
-    PP_HIP(hipMemcpyAsync(R.data(), d_res, resb, hipMemcpyDeviceToHost, ctx->stream));
-    PP_HIP(hipMemcpyAsync(st.data(), d_st, stb, hipMemcpyDeviceToHost, ctx->stream));
-    if (mask && !mask_on_device) PP_HIP(hipMemcpyAsync(mask, d_mask, N, hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
-    unpack_results(R, st, pp, out);
-    return rc;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->timing.shell_ms = ms;
+    PP_CHECK(copy_results(ctx, S, out));
+    if (mask && !mask_on_device) {
+        PP_HIP(hipMemcpyAsync(mask, d_mask, N, hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return PP_OK;
 }
 
 int pp_shell_profiles(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks, double alatt,
@@ -468,42 +541,46 @@ int pp_shell_profiles(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
     const int64_t np = peaks->npeaks;
     if (np == 0) return PP_OK;
     const int n = fields->n1;
-    std::vector<PeakParam> pp;
-    PP_CHECK(build_peak_params(ctx, peaks, n, alatt, rmax2rs, nullptr, pp));
     GatherFields G;
     PP_CHECK(upload_fields(ctx, fields, G));
     PostConst C;
     PP_CHECK(prepare_post_const(ctx, n, 0, alatt, G.lap, nullptr, C));
-    const size_t resb = (size_t)np * R_NREC * sizeof(double), stb = (size_t)np * sizeof(int32_t);
-    const size_t ppb = (size_t)np * sizeof(PeakParam);
-    PP_CHECK(ctx->d_res.grow(resb + stb + ppb + 1024));
-    double *d_res = ctx->d_res.as<double>();
-    int32_t *d_st = (int32_t *)((char *)d_res + resb);
-    PeakParam *d_pp = (PeakParam *)((char *)d_st + ((stb + 255) & ~(size_t)255));
-    PP_HIP(hipMemcpyAsync(d_pp, pp.data(), ppb, hipMemcpyHostToDevice, ctx->stream));
-    int64_t tot = 0;
+    ShellRun S;
+    PP_CHECK(prepare_batch(ctx, peaks, n, alatt, rmax2rs, nullptr, S));
+    std::vector<PeakParam> hp(np);
+    PP_HIP(hipMemcpy(hp.data(), S.pp, np * sizeof(PeakParam), hipMemcpyDeviceToHost));
+    std::vector<int32_t> cnt;
+    std::vector<double> fb;
     std::vector<int64_t> off(np);
-    for (int64_t p = 0; p < np; p++) { off[p] = tot; tot += pp[p].m; }
-    PP_ARG(tot * (int64_t)(PF_NFIELD * 4 + 8) <= ((int64_t)1 << 31), "profile export batch too large");
-    // the plain kernel keeps every profile of a single chunk in d_prof
-    PP_CHECK(run_plain(ctx, G, pp, d_pp, C, d_res, d_st));
-    char *base = ctx->d_prof.as<char>();
-    float *d_f = (float *)(base + (((size_t)np * 8 + 255) & ~(size_t)255));
-    double *d_fb = (double *)((char *)d_f + (((size_t)tot * PF_NFIELD * 4 + 255) & ~(size_t)255));
-    std::vector<float> hf((size_t)tot * PF_NFIELD);
-    std::vector<double> hfb((size_t)tot);
-    PP_HIP(hipMemcpy(hf.data(), d_f, hf.size() * 4, hipMemcpyDeviceToHost));
-    PP_HIP(hipMemcpy(hfb.data(), d_fb, hfb.size() * 8, hipMemcpyDeviceToHost));
+    if (kernel_form == PP_SHELL_PERSISTENT) {
+        PP_CHECK(persistent_profiles(ctx, G, S, C, cnt, fb, off));
+    } else {
+        std::vector<int64_t> o;
+        PP_CHECK(run_plain(ctx, G, S, C, &o));
+        int64_t tot = 0;
+        for (int64_t p = 0; p < np; p++) tot += hp[p].m;
+        char *base = ctx->d_prof.as<char>();
+        const size_t offb = (((size_t)np * 8) + 255) & ~(size_t)255;
+        const size_t fbb = (((size_t)tot * PF_NFIELD * 4) + 255) & ~(size_t)255;
+        std::vector<float> hf((size_t)tot * PF_NFIELD);
+        fb.resize(tot);
+        PP_HIP(hipMemcpy(hf.data(), base + offb, hf.size() * 4, hipMemcpyDeviceToHost));
+        PP_HIP(hipMemcpy(fb.data(), base + offb + fbb, fb.size() * 8, hipMemcpyDeviceToHost));
+        cnt.resize(tot);
+        for (int64_t p = 0; p < np; p++) {
+            const int m = hp[p].m;
+            for (int s = 0; s < m; s++) cnt[o[p] + s] = (int32_t)hf[(size_t)o[p] * PF_NFIELD + PF_N * m + s];
+        }
+        off = o;
+    }
     for (int64_t p = 0; p < np; p++) {
-        const int m = pp[p].m;
+        const int m = hp[p].m;
         m_out[p] = m;
-        const float *f = &hf[(size_t)off[p] * PF_NFIELD];
         for (int s = 0; s < max_shells; s++) {
-            if (nshell_out) nshell_out[p * max_shells + s] = s < m ? (int32_t)f[PF_N * m + s] : 0;
-            if (Fbar_out) Fbar_out[p * max_shells + s] = s < m ? hfb[(size_t)off[p] + s] : 0.0;
+            if (nshell_out) nshell_out[p * max_shells + s] = s < m ? cnt[off[p] + s] : 0;
+            if (Fbar_out) Fbar_out[p * max_shells + s] = s < m ? fb[off[p] + s] : 0.0;
         }
     }
-    (void)kernel_form;
     return PP_OK;
 }
 

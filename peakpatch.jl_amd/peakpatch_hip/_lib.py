@@ -40,7 +40,8 @@ class Fields(C.Structure):
 
 class PeakBatch(C.Structure):
     _fields_ = [("npeaks", C.c_int64), ("i", C.c_void_p), ("j", C.c_void_p), ("k", C.c_void_p),
-                ("Rf", C.c_void_p), ("ir2min", C.c_void_p), ("ZZon", C.c_void_p), ("fcrit", C.c_void_p)]
+                ("Rf", C.c_void_p), ("ir2min", C.c_void_p), ("ZZon", C.c_void_p), ("fcrit", C.c_void_p),
+                ("on_device", C.c_int)]
 
 
 class Results(C.Structure):
@@ -48,7 +49,8 @@ class Results(C.Structure):
                 ("strain", C.c_void_p), ("gradpk", C.c_void_p), ("gradpkrf", C.c_void_p),
                 ("Fbarx", C.c_void_p), ("e_v", C.c_void_p), ("p_v", C.c_void_p),
                 ("zvir_half", C.c_void_p), ("gradpkf", C.c_void_p), ("d2F", C.c_void_p),
-                ("status", C.c_void_p), ("nshells", C.c_void_p), ("dump_counts", C.c_int64 * 6)]
+                ("status", C.c_void_p), ("nshells", C.c_void_p), ("on_device", C.c_int),
+                ("dump_counts", C.c_int64 * 7)]
 
 
 _vp, _i, _d, _f, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int64

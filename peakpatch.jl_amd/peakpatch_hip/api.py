@@ -204,6 +204,17 @@ class Context:
 
     @staticmethod
     def _batch(i, j, k, Rf, ir2min, ZZon, fcrit):
+        """Host numpy batch, or a device batch when i is a CUDA tensor (then every
+        per-peak array must be a CUDA tensor of the right dtype)."""
+        if _is_torch(i) and i.is_cuda:
+            import torch
+            n = i.numel()
+            want = (torch.int32, torch.int32, torch.int32, torch.float64, torch.int64, torch.float64, torch.float64)
+            arrs = [i, j, k, Rf, ir2min, ZZon, fcrit]
+            for a, w in zip(arrs, want):
+                if not (_is_torch(a) and a.is_cuda and a.dtype == w and a.is_contiguous() and a.numel() == n):
+                    raise ValueError("device batch: i, j, k int32; Rf float64; ir2min int64; ZZon, fcrit float64")
+            return L.PeakBatch(n, *(a.data_ptr() for a in arrs), 1), arrs
         n = len(i)
         arrs = [np.ascontiguousarray(i, np.int32), np.ascontiguousarray(j, np.int32),
                 np.ascontiguousarray(k, np.int32),
@@ -211,23 +222,32 @@ class Context:
                 np.ascontiguousarray(np.broadcast_to(ir2min, (n,)), np.int64),
                 np.ascontiguousarray(np.broadcast_to(ZZon, (n,)), np.float64),
                 np.ascontiguousarray(np.broadcast_to(fcrit, (n,)), np.float64)]
-        return L.PeakBatch(n, *(a.ctypes.data for a in arrs)), arrs
+        return L.PeakBatch(n, *(a.ctypes.data for a in arrs), 0), arrs
 
     def analyse_peaks(self, delta, eta, i, j, k, Rf, ir2min, ZZon, fcrit, alatt: float, eta2=None,
                       lapd=None, rmax2rs: float = 0.0, nbuff: int = 0, mask=None,
-                      kernel: int = L.SHELL_PLAIN) -> dict:
+                      kernel: int = L.SHELL_PLAIN, device_out: bool = False) -> dict:
         """analyse_peak over a batch (RadialShell.jl:282-891).  Returns a dict
-        of numpy arrays keyed as PeakResult plus 'status', 'nshells',
-        'dump_counts'.  mask (int8 numpy or device tensor) is updated."""
+        of arrays keyed as PeakResult plus 'status', 'nshells' and
+        'dump_counts'; numpy on the host, or CUDA tensors with device_out.
+        mask (int8 numpy or device tensor) is updated."""
         f, keep = self._fields(delta, eta, eta2, lapd)
         b, keepb = self._batch(i, j, k, Rf, ir2min, ZZon, fcrit)
-        n = len(keepb[0])
-        out = {name: np.zeros((n, 3) if name in _VEC3 else ((n, 3, 3) if name == "strain" else n), np.float64)
-               for name in RESULT_FIELDS}
-        out["status"] = np.zeros(n, np.int32)
-        out["nshells"] = np.zeros(n, np.int32)
-        r = L.Results(*(out[name].ctypes.data for name in RESULT_FIELDS), out["status"].ctypes.data,
-                      out["nshells"].ctypes.data)
+        n = int(b.npeaks)
+        shapes = {name: (n, 3) if name in _VEC3 else ((n, 9) if name == "strain" else (n,)) for name in RESULT_FIELDS}
+        if device_out:
+            import torch
+            out = {name: torch.empty(shp, dtype=torch.float64, device="cuda") for name, shp in shapes.items()}
+            out["status"] = torch.empty(n, dtype=torch.int32, device="cuda")
+            out["nshells"] = torch.empty(n, dtype=torch.int32, device="cuda")
+            addr = lambda a: a.data_ptr()
+        else:
+            out = {name: np.zeros(shp, np.float64) for name, shp in shapes.items()}
+            out["status"] = np.zeros(n, np.int32)
+            out["nshells"] = np.zeros(n, np.int32)
+            addr = lambda a: a.ctypes.data
+        r = L.Results(*(addr(out[name]) for name in RESULT_FIELDS), addr(out["status"]), addr(out["nshells"]),
+                      1 if device_out else 0)
         mp, mdev = None, 0
         if mask is not None:
             if _is_torch(mask):
@@ -238,9 +258,12 @@ class Context:
                 mp = mask.ctypes.data
         L.check(self.lib.pp_analyse_peaks(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, nbuff, mp, mdev,
                                           kernel, C.byref(r)))
-        # strain was written column-major per peak: [p, L + 3K] -> (p, L, K)
-        out["strain"] = out["strain"].reshape(n, 9).reshape(n, 3, 3, order="F")
-        out["dump_counts"] = dict(zip(DUMP_NAMES[:6], list(r.dump_counts)))
+        # strain is column-major per peak: [p, L + 3K] -> (p, L, K)
+        if device_out:
+            out["strain"] = out["strain"].reshape(n, 3, 3).transpose(1, 2)
+        else:
+            out["strain"] = out["strain"].reshape(n, 3, 3, order="C").transpose(0, 2, 1)
+        out["dump_counts"] = dict(zip(DUMP_NAMES, list(r.dump_counts)))
         return out
 
     def shell_profiles(self, delta, eta, i, j, k, Rf, alatt: float, eta2=None, lapd=None,

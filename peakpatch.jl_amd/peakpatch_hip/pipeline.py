@@ -110,6 +110,14 @@ class PeakBatch:
     def take(self, sel):
         return PeakBatch(*(getattr(self, f)[sel] for f in self.__dataclass_fields__))
 
+    def to_device(self, device="cuda") -> dict:
+        """The per-peak arrays analyse_peaks reads, as CUDA tensors."""
+        import torch
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+        return {"i": t(self.i, torch.int32), "j": t(self.j, torch.int32), "k": t(self.k, torch.int32),
+                "Rf": t(self.Rf, torch.float64), "ir2min": t(self.ir2min, torch.int64),
+                "ZZon": t(self.ZZon, torch.float64), "fcrit": t(self.fcrit, torch.float64)}
+
 
 class TileDriver:
     """Holds the per-process host state a tile needs (growth tables, chi->z)."""
@@ -154,12 +162,15 @@ class TileDriver:
             b = b.take(zpk <= c.z_max)
         return b
 
-    def analyse(self, ctx, fields, b: PeakBatch, mask=None, kernel=SHELL_PERSISTENT):
+    def analyse(self, ctx, fields, b, mask=None, kernel=SHELL_PERSISTENT, device_out=False):
+        """b: a PeakBatch (host) or the dict of CUDA tensors of PeakBatch.to_device()."""
         c = self.cfg
         delta, eta, eta2, lapd = fields
-        return ctx.analyse_peaks(delta, eta, b.i, b.j, b.k, b.Rf, b.ir2min, b.ZZon, b.fcrit, c.alatt,
-                                 eta2=eta2 if c.ilpt >= 2 else None, lapd=lapd if c.ioutshear >= 1 else None,
-                                 rmax2rs=c.rmax2rs, nbuff=c.nbuff_eff, mask=mask, kernel=kernel)
+        g = (lambda f: b[f]) if isinstance(b, dict) else (lambda f: getattr(b, f))
+        return ctx.analyse_peaks(delta, eta, g("i"), g("j"), g("k"), g("Rf"), g("ir2min"), g("ZZon"), g("fcrit"),
+                                 c.alatt, eta2=eta2 if c.ilpt >= 2 else None,
+                                 lapd=lapd if c.ioutshear >= 1 else None, rmax2rs=c.rmax2rs, nbuff=c.nbuff_eff,
+                                 mask=mask, kernel=kernel, device_out=device_out)
 
     def gather_cells(self, b: PeakBatch) -> np.ndarray:
         """Cells each peak's gather reads (centre + finalised shells), the unit
