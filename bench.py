@@ -54,7 +54,7 @@ def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
         mask.zero_()
         torch.cuda.synchronize()
         drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
-    walls, kern, gath = [], [], []
+    walls, kern, stages = [], [], []
     res = None
     for _ in range(steps):
         mask.zero_()
@@ -65,9 +65,9 @@ def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
         walls.append(time.perf_counter() - t0)
         tm = ctx.last_timing()
         kern.append(tm["shell_ms"] / 1e3)
-        gath.append(tm["gather_ms"] / 1e3)
+        stages.append(tm["stages_ms"])
     res = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in res.items()}
-    return res, walls, kern, gath
+    return res, walls, kern, stages
 
 
 def cpu_baseline(cfg, tile, b, ct, seconds: float):
@@ -116,6 +116,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-peak form")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -149,6 +150,15 @@ def main():
     prop = torch.cuda.get_device_properties(0)
     log(f"[bench] device {prop.name}: {prop.multi_processor_count} CUs, {prop.total_memory / 2**30:.0f} GiB")
 
+    tuned = ctx.tunables()
+    if not args.no_autotune and args.form in ("auto", "persistent"):
+        t0 = time.perf_counter()
+        c = cfg
+        tuned = ctx.autotune_shell(fields[0], fields[1], bdev["i"], bdev["j"], bdev["k"], bdev["Rf"], bdev["ir2min"],
+                                   bdev["ZZon"], bdev["fcrit"], c.alatt, eta2=fields[2] if c.ilpt >= 2 else None,
+                                   lapd=fields[3] if c.ioutshear >= 1 else None, rmax2rs=c.rmax2rs)
+        log(f"[bench] autotuned launch parameters in {time.perf_counter() - t0:.1f} s: {tuned}")
+
     forms = []
     if args.form in ("auto", "persistent"):
         forms.append(("persistent", SHELL_PERSISTENT))
@@ -158,23 +168,34 @@ def main():
     results = {}
     for name, form in forms:
         try:
-            res, walls, kern, gath = time_form(drv, ctx, fields, bdev, mask, form, args.steps, args.warmup)
+            res, walls, kern, stages = time_form(drv, ctx, fields, bdev, mask, form, args.steps, args.warmup)
         except PPError as e:
             log(f"[bench] {name} form unavailable: {e}")
             continue
-        timings[name] = (walls, kern, gath)
+        timings[name] = (walls, kern, stages)
         results[name] = res
-        log(f"[bench] {name}: wall {np.median(walls) * 1e3:.1f} ms/step, device {np.median(kern) * 1e3:.1f} ms "
-            f"(gather {np.median(gath) * 1e3:.1f} ms), collapsed {(res['RTHL'] > 0).sum()}, dumps {res['dump_counts']}")
+        log(f"[bench] {name}: wall {np.median(walls) * 1e3:.1f} ms/step, device {np.median(kern) * 1e3:.1f} ms, "
+            f"stages {stages[-1]}, collapsed {(res['RTHL'] > 0).sum()}, dumps {res['dump_counts']}")
     head = "persistent" if "persistent" in timings else "plain"
-    walls, kern, gath = timings[head]
+    walls, kern, stages = timings[head]
     npk = len(b)
     ms_step = float(np.mean(walls)) * 1e3
     k_s = float(np.mean(kern))
-    cells = drv.gather_cells(b)
+    cells, mshell = drv.gather_extent(b)
     F = nfields(cfg)
-    alg_bytes = float(4 * F * cells.sum())
-    achieved = alg_bytes / k_s / 1e9
+    stage_ms = {k: float(np.mean([st[k] for st in stages])) for k in stages[0]}
+    if head == "persistent":
+        # dominant kernel: the wave gather (k_gather) over the peaks above the
+        # small-peak threshold; bytes = 4 B x F fields x cells it reads
+        big = mshell > tuned["small_m"]
+        dom_bytes = float(4 * F * cells[big].sum())
+        dom_s = stage_ms["gather"] / 1e3
+        dom_name = "k_gather"
+    else:
+        dom_bytes = float(4 * F * cells.sum())
+        dom_s = k_s
+        dom_name = "k_shell_plain"
+    achieved = dom_bytes / dom_s / 1e9
     out = {
         "metric": METRIC,
         "value": npk / (ms_step / 1e3),
@@ -191,9 +212,12 @@ def main():
         "config": {"workload": cfg.describe(), "npeaks": npk, "kernel_form": head,
                    "collapse_table": table_name, "fields_F": F, "cells_gathered": int(cells.sum())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_name,
+                     "kernel_ms": dom_s * 1e3, "algorithmic_bytes": dom_bytes},
         "kernel_ms_per_step": k_s * 1e3,
-        "gather_ms_per_step": float(np.mean(gath)) * 1e3,
+        "stages_ms": stage_ms,
+        "tunables": tuned,
+        "shell_algorithmic_GBps": float(4 * F * cells.sum()) / k_s / 1e9,
         "kernel_peaks_per_s": npk / k_s,
         "peakfind_ms": pf_ms,
         "halos": int((results[head]["RTHL"] > 0).sum()),

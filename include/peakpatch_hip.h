@@ -185,16 +185,37 @@ int pp_shell_profiles(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
                       double alatt, double rmax2rs, int kernel_form, int max_shells,
                       int32_t *m_out, int32_t *nshell_out, double *Fbar_out);
 
-/* Autotuner: times the persistent kernel on the given batch for each
- * candidate launch shape and keeps the fastest per radius class in ctx.
- * Returns the chosen workgroups-per-CU in *choice (may be NULL). */
+/* Autotuner: times the persistent form on the given batch over its launch
+ * parameters (the pp_shell_tunables below, one at a time) and keeps the
+ * fastest setting in ctx.  Returns the chosen gather waves per CU in *choice
+ * (may be NULL).  Counterpart of the reference's per-radius-class launch
+ * heuristics (ext/CUDAExt.jl:2246-2462, src/GPUBackend.jl). */
 int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks,
                       double alatt, double rmax2rs, int *choice);
+
+/* Launch parameters of the persistent form, in this order:
+ *   [0] small-peak threshold in finalised shells (peaks with m at or below it
+ *       are packed several per wave; 0 disables that path; at most 16),
+ *   [1] shells per LDS chunk of the wave gather (8..1024),
+ *   [2] shell count above which a peak's Float64 post runs on a whole wave,
+ *   [3] cap on resident gather waves per CU.
+ * set (may be NULL): new values, entries <= 0 keep the current one except
+ * [0], where 0 is a value and < 0 keeps; get (may be NULL): the values in
+ * force.  They change speed only: results do not depend on them. */
+#define PP_NTUNABLES 4
+int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get);
 
 /* Kernel timing of the last pp_analyse_peaks / pp_find_peaks call (HIP
  * events on the context stream), milliseconds. */
 int pp_last_timing(pp_ctx *ctx, double *shell_gather_ms, double *shell_total_ms,
                    double *peakfind_ms);
+
+/* Per-stage device time of the last persistent-form call, milliseconds
+ * (HIP events on the context stream): ms[0] preparation (transpose, order,
+ * offsets), ms[1] wave gather, ms[2] small-peak gather, ms[3] wave post,
+ * ms[4] lane posts, ms[5] finish.  No counterpart in the reference (its
+ * profile=true stage timers, MultiResolution.jl, time whole stages). */
+int pp_last_shell_stages(pp_ctx *ctx, double *ms);
 
 #ifdef __cplusplus
 }

@@ -90,12 +90,20 @@ struct ShellTable {
     std::vector<int64_t> class_groups; // groups with r^2 <= class bound (prefix)
     DevBuf d_rad;         // double [nshells], sqrt(r2)
     DevBuf d_akk;         // double [1101], atab4
+    // kernel_strain weights w(mp, m1) for every shell pair the radial kernel
+    // touches: row mp-1 of d_wrow = {offset, first m1, last m1, 0}
+    DevBuf d_wrow;        // int4 [nshells]
+    DevBuf d_wtab;        // double
+    DevBuf d_shell_ge;    // int32 [max r2 + 2]: first shell with r2 >= k
     int rad_rmax = -1;
 };
 
 // ------------------------------------------------------------ context
 struct Timing {
     double gather_ms = 0, shell_ms = 0, peakfind_ms = 0;
+    // persistent form, per stage: prepare (transpose, order, scan), k_gather,
+    // k_small, wave post, lane posts, finish
+    double stage_ms[6] = {0, 0, 0, 0, 0, 0};
 };
 
 struct FftPlans {
@@ -123,17 +131,23 @@ struct pp_ctx {
     // scratch
     pp::DevBuf d_field_a, d_field_b, d_k0, d_k1, d_k2, d_mask, d_misc, d_peaks, d_scan, d_cub;
     pp::DevBuf d_in[8], d_pk, d_res, d_prof, d_queue;
-    // persistent-form tunables (autotuned): tiny-peak threshold in finalised
-    // shells, and the cap on gather waves per CU
+    // persistent-form tunables (pp_autotune_shell): small-peak threshold in
+    // finalised shells (several peaks per wave), shells per LDS chunk of the
+    // wave gather, the shell count above which a peak's Float64 post runs on
+    // a whole wave instead of one lane, and the cap on gather waves per CU
     int tiny_m = 16;
+    int chunk_shells = 64;
+    int coop_m = 64;
     int gather_waves_cu = 16;
-    int wg_per_cu = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // events: [0,1] whole call, [2..8] persistent-form stage boundaries
+    static constexpr int NEV = 9;
+    hipEvent_t ev[NEV] = {};
     pp::Timing timing;
 };
 
 namespace pp {
 int fft_prepare(pp_ctx *ctx, int n);
 int shell_groups_build(pp_ctx *ctx);  // pp_persist.hip
+void persist_release(pp_ctx *ctx);    // pp_persist.hip: drops the context's gather tables
 int ensure_device(pp_ctx *ctx);
 }  // namespace pp

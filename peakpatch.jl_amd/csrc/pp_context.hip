@@ -112,7 +112,7 @@ int pp_ctx_create(int device, pp_ctx **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
-    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < pp_ctx::NEV && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) {
         set_error("context creation on device %d failed: %s", device, hipGetErrorString(e));
         delete c;
@@ -138,8 +138,11 @@ int pp_ctx_destroy(pp_ctx *ctx)
                       &ctx->d_k2, &ctx->d_mask, &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan,
                       &ctx->d_cub, &ctx->d_pk, &ctx->d_res, &ctx->d_prof, &ctx->d_queue,
                       &ctx->shells.d_cell, &ctx->shells.d_shell_start, &ctx->shells.d_shell_r2,
-                      &ctx->shells.d_shell_of_r2, &ctx->shells.d_groups};
+                      &ctx->shells.d_shell_of_r2, &ctx->shells.d_groups, &ctx->shells.d_rad,
+                      &ctx->shells.d_akk, &ctx->shells.d_wrow, &ctx->shells.d_wtab,
+                      &ctx->shells.d_shell_ge};
     for (DevBuf *b : bufs) b->release();
+    persist_release(ctx);
     for (auto &b : ctx->d_in) b.release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -254,6 +257,34 @@ int pp_shell_table_info(pp_ctx *ctx, int64_t *ncells, int32_t *nshells)
     }
     *ncells = ctx->shells.ncells;
     *nshells = ctx->shells.nshells;
+    return PP_OK;
+}
+
+int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get)
+{
+    PP_ARG(ctx, "null context");
+    if (set) {
+        if (set[0] > 16) { set_error("small-peak threshold above 16 shells"); return PP_ERR_ARG; }
+        if (set[1] > 0 && (set[1] < 8 || set[1] > 1024)) { set_error("chunk shells outside [8, 1024]"); return PP_ERR_ARG; }
+        if (set[3] > 64) { set_error("more than 64 gather waves per CU"); return PP_ERR_ARG; }
+        if (set[0] >= 0) ctx->tiny_m = set[0];
+        if (set[1] > 0) ctx->chunk_shells = set[1];
+        if (set[2] > 0) ctx->coop_m = set[2];
+        if (set[3] > 0) ctx->gather_waves_cu = set[3];
+    }
+    if (get) {
+        get[0] = ctx->tiny_m;
+        get[1] = ctx->chunk_shells;
+        get[2] = ctx->coop_m;
+        get[3] = ctx->gather_waves_cu;
+    }
+    return PP_OK;
+}
+
+int pp_last_shell_stages(pp_ctx *ctx, double *ms)
+{
+    PP_ARG(ctx && ms, "NULL argument");
+    for (int q = 0; q < 6; q++) ms[q] = ctx->timing.stage_ms[q];
     return PP_OK;
 }
 

@@ -4,29 +4,33 @@
 //   1. k_transpose: dT(x,y,z) = delta(z,x,y), so the reference's permuted read
 //      delta[iv3, iv1, iv2] (RadialShell.jl:448) coalesces like the others.
 //   2. peaks are ordered by finalised-shell count m, largest first (already
-//      the case for the peak finder's output, else a radix sort).
-//   3. tiny peaks (m <= tiny_m): k_tiny, one lane per peak: gather, Fbar,
-//      virialisation walk and finish, all lane-serial (64 peaks per wave).
-//   4. other peaks: k_gather, a persistent work-queue kernel (one wave per
-//      peak, peaks pulled from an atomic counter in m order).  The peak's
-//      sphere is read as octant groups (|di|, |dj|, |dk|) in row-major order,
-//      so the lanes of a wave read contiguous x-runs of 5 fields (delta, dT,
-//      eta x/y/z); each group's <= 8 images are summed in registers and
-//      deposited into per-shell LDS bins with Neumaier-compensated FP32
-//      adds.  The host packs the groups into 64-lane batches in which no two
-//      groups share a shell, so the deposits need no atomics and the sums
-//      are deterministic.  The finished profile (16 floats per shell) goes to
-//      global memory.
-//   5. k_post_lane: one lane per peak runs the Float64 post-process
-//      (Fbar, fcrit crossing, walk with strain eigenvalues and collapse-table
-//      lookups, R_TH, Srb, zvir_half) on that profile.
-//   6. k_finish: one wave per collapsed peak sums eta, eta2 over the shells
-//      inside m0 and lapd over the cells inside R_TH (the only cells those
-//      outputs use) and writes the mask.
-// Peaks whose profile would not fit the LDS bins use the plain kernel.
+//      the case for the peak finder's output, else a radix sort), and every
+//      peak gets a profile of m 24-float shell records (exclusive scan of m).
+//   3. gather.  A peak's sphere is read as octant groups (|di|, |dj|, |dk|):
+//      one lane takes a group and sums its <= 8 mirror images in registers.
+//      * small peaks (m <= tiny_m, at most 64 groups): k_small packs
+//        floor(64 / groups) peaks of equal m into one wave, stages the group
+//        sums in LDS and reduces them per shell in a fixed order.
+//      * other peaks: k_gather, a persistent work-queue kernel (one wave per
+//        peak, peaks pulled from an atomic counter in m order).  The shells
+//        are processed in chunks of chunk_shells; within a chunk the host
+//        packed the groups (row-major, so lanes read neighbouring x-runs)
+//        into 64-lane batches in which no two groups share a shell, so each
+//        lane deposits into per-shell LDS bins with Neumaier-compensated
+//        FP32 adds, without atomics and in a fixed order.  The chunking keeps
+//        the LDS per wave small, so occupancy does not fall with the radius.
+//   4. Float64 post-process (Fbar, fcrit crossing, walk with strain
+//      eigenvalues and collapse-table lookups, R_TH, Srb, zvir_half):
+//      k_post_wave (one wave per peak, radial sums split over the lanes) for
+//      m > coop_m, k_post_lane (one lane per peak) below.
+//   5. k_finish: one wave per collapsed non-small peak sums eta, eta2 over the
+//      shells inside m0 and lapd over the cells inside R_TH and writes the
+//      mask; small peaks carry those sums in their profile, so their lane
+//      post finishes them.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -35,8 +39,15 @@
 
 namespace pp {
 
-constexpr int PR = 16;  // floats per shell record of the AoS profile
-enum : int { P_F = 0, P_N = 1, P_G = 2, P_GF = 5, P_E = 8, P_NB = 14, P_FB = 14 /* Fbar: double in slots 14-15 */ };
+constexpr int PR = 24;  // floats per shell record of the AoS profile
+// record: G(3) Gf(3) E(6) | F, count | Fbar (double) | S(3) S2(3) lapd | pad;
+// the 12 strain inputs load as three 16-byte vectors.  S, S2, lapd are filled
+// for small peaks only.
+enum : int {
+    P_G = 0, P_GF = 3, P_E = 6, P_F = 12, P_N = 13, P_NB = 14 /* fields binned by k_gather */,
+    P_FB = 14 /* Fbar: double in slots 14-15 */, P_S = 16, P_S2 = 19, P_L = 22
+};
+constexpr int SMALL_MAX = 16;  // largest m whose groups fit one 64-lane batch (62 groups)
 
 // ------------------------------------------------------------- transpose
 __global__ void __launch_bounds__(256) k_transpose(const float *__restrict__ in, float *__restrict__ out, int n)
@@ -105,134 +116,195 @@ __global__ void k_fill_i32b(int *p, int64_t n, int v)
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
 }
 
-// ---------------------------------------------------------- tiny peaks
-// Lane-private profiles in an interleaved scratch: element (field f, shell s)
-// of lane L lives at f*TM*NL + (s-1)*NL + L, so a wave's lanes, which walk
-// the shells in step, store contiguously.
-struct LaneProfile {
-    const float *f;
-    const double *fb;
-    const double *radt;
-    int TM;
-    int64_t NL, L;
-    bool lapd;
-    __device__ float at(int fld, int s) const { return f[((int64_t)fld * TM + (s - 1)) * NL + L]; }
-    __device__ double rad(int s) const { return radt[s - 1]; }
-    __device__ double Fbar(int s) const { return fb[(int64_t)(s - 1) * NL + L]; }
-    __device__ double count(int s) const { return (double)at(PF_N, s); }
-    __device__ double S(int s, int q) const { return (double)at(PF_S + q, s); }
-    __device__ double S2(int s, int q) const { return (double)at(PF_S2 + q, s); }
-    __device__ double lap(int s) const { return (double)at(PF_L, s); }
-    __device__ bool has_lapd() const { return lapd; }
-    __device__ void strain_inputs(int s, double G[3], double Gf[3], double Es[6]) const
-    {
-        if (s == 1) {
-            for (int i = 0; i < 3; i++) G[i] = Gf[i] = 0.0;
-            for (int i = 0; i < 6; i++) Es[i] = 0.0;
-            return;
-        }
-        const double r = radt[s - 1];
-        for (int i = 0; i < 3; i++) {
-            G[i] = (double)at(PF_G + i, s) / r;
-            Gf[i] = (double)at(PF_GF + i, s) / r;
-        }
-        double SR[9];
-        for (int i = 0; i < 9; i++) SR[i] = (double)at(PF_SR + i, s) / r;
-        Es[0] = SR[0] + SR[0]; Es[1] = SR[4] + SR[4]; Es[2] = SR[8] + SR[8];
-        Es[3] = SR[1] + SR[3]; Es[4] = SR[2] + SR[6]; Es[5] = SR[5] + SR[7];
-    }
-};
-
-__global__ void __launch_bounds__(64) k_tiny(GatherFields F, const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
-                                             float *__restrict__ scratch, double *__restrict__ fbs, int TM, PostConst C,
-                                             ResultView R)
-{
-    const int64_t NL = (int64_t)gridDim.x * blockDim.x;
-    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n1 = F.n, n12 = (long)F.n * F.n;
-    for (int64_t idx = lo + L; idx < hi; idx += NL) {
-        const PeakParam pk = pps[idx];
-        const int m = pk.m;
-        const bool inside = pk.flags & 1;
-        auto put = [&](int fld, int s, float v) { scratch[((int64_t)fld * TM + (s - 1)) * NL + L] = v; };
-        {
-            const long c = pk.ci + pk.cj * n1 + pk.ck * n12;
-            put(PF_N, 1, 1.0f);
-            put(PF_F, 1, F.d[c]);
-            put(PF_S + 0, 1, F.ex[c]);
-            put(PF_S + 1, 1, F.ey[c]);
-            put(PF_S + 2, 1, F.ez[c]);
-            put(PF_S2 + 0, 1, F.e2x ? F.e2x[c] : 0.0f);
-            put(PF_S2 + 1, 1, F.e2x ? F.e2y[c] : 0.0f);
-            put(PF_S2 + 2, 1, F.e2x ? F.e2z[c] : 0.0f);
-            for (int q = PF_G; q < PF_NFIELD; q++) put(q, 1, 0.0f);
-        }
-        for (int s = 2; s <= m; s++) {
-            float a[PF_NFIELD];
-#pragma unroll
-            for (int q = 0; q < PF_NFIELD; q++) a[q] = 0.0f;
-            const int c0 = C.shell_start[s - 1], c1 = C.shell_start[s];
-            for (int c = c0; c < c1; c++) {
-                const int pc = C.cell[c];
-                const int di = (pc & 255) - 128, dj = ((pc >> 8) & 255) - 128, dk = ((pc >> 16) & 255) - 128;
-                const int x = pk.ci + di, y = pk.cj + dj, z = pk.ck + dk;
-                if (!inside && (x < 0 || x >= F.n || y < 0 || y >= F.n || z < 0 || z >= F.n)) continue;
-                const long a0 = x + y * n1 + z * n12;
-                const float d = F.d[a0], dT = F.d[z + x * n1 + y * n12];
-                const float ex = F.ex[a0], ey = F.ey[a0], ez = F.ez[a0];
-                const float fx = (float)di, fy = (float)dj, fz = (float)dk;
-                a[PF_N] += 1.0f;
-                a[PF_F] += d;
-                a[PF_S + 0] += ex; a[PF_S + 1] += ey; a[PF_S + 2] += ez;
-                if (F.e2x) { a[PF_S2 + 0] += F.e2x[a0]; a[PF_S2 + 1] += F.e2y[a0]; a[PF_S2 + 2] += F.e2z[a0]; }
-                a[PF_G + 0] += d * fx; a[PF_G + 1] += d * fy; a[PF_G + 2] += d * fz;
-                a[PF_GF + 0] += dT * fx; a[PF_GF + 1] += dT * fy; a[PF_GF + 2] += dT * fz;
-                a[PF_SR + 0] += ex * fx; a[PF_SR + 1] += ex * fy; a[PF_SR + 2] += ex * fz;
-                a[PF_SR + 3] += ey * fx; a[PF_SR + 4] += ey * fy; a[PF_SR + 5] += ey * fz;
-                a[PF_SR + 6] += ez * fx; a[PF_SR + 7] += ez * fy; a[PF_SR + 8] += ez * fz;
-                if (F.lap) a[PF_L] += F.lap[a0];
-            }
-#pragma unroll
-            for (int q = 0; q < PF_NFIELD; q++) put(q, s, a[q]);
-        }
-        // Fbar recursion (RadialShell.jl:383-390)
-        {
-            double F1 = (double)scratch[((int64_t)PF_F * TM) * NL + L];
-            fbs[L] = F1;
-            double fbp = F1, dFbarp = F1;
-            for (int s = 2; s <= m; s++) {
-                const double nn = (double)scratch[((int64_t)PF_N * TM + (s - 1)) * NL + L];
-                const double dF = nn > 0.0 ? (double)scratch[((int64_t)PF_F * TM + (s - 1)) * NL + L] / nn : 0.0;
-                const double rp = C.rad[s - 2], r = C.rad[s - 1];
-                const double r3p = rp * rp * rp, r3 = r * r * r;
-                fbp = (r3p * fbp + 0.5 * (dFbarp + dF) * (r3 - r3p)) / r3;
-                fbs[(int64_t)(s - 1) * NL + L] = fbp;
-                dFbarp = dF;
-            }
-        }
-        LaneProfile P{scratch, fbs, C.rad, TM, NL, L, F.lap != nullptr};
-        analyse_post<false, true>(P, pk, C, R, pk.order);
-        R.nshells[pk.order] = m;
-    }
-}
-
-// ----------------------------------------------------------- wave gather
+// ----------------------------------------------------------- group sums
 struct GroupDesc {
     int32_t lmn;  // l | dj << 8 | dk << 16
     int32_t s;    // shell (1-based), -1 for an empty slot
 };
 
+// Sum of one octant group's mirror images (those inside the grid) into acc,
+// indexed by record slot.  FULL adds S, S2 and lapd.
+template <bool FULL>
+__device__ __forceinline__ void group_sum(const GatherFields &F, const PeakParam &pk, GroupDesc g, float acc[PR])
+{
+    const long n1 = F.n, n12 = (long)F.n * F.n;
+    const bool inside = pk.flags & 1;
+    const int l = g.lmn & 255, dj = (g.lmn >> 8) & 255, dk = (g.lmn >> 16) & 255;
+#pragma unroll
+    for (int img = 0; img < 8; img++) {
+        const int sx = (img & 1) ? -1 : 1, sy = (img & 2) ? -1 : 1, sz = (img & 4) ? -1 : 1;
+        if ((sx < 0 && l == 0) || (sy < 0 && dj == 0) || (sz < 0 && dk == 0)) continue;
+        const int dx = sx * l, dy = sy * dj, dz = sz * dk;
+        const int x = pk.ci + dx, y = pk.cj + dy, z = pk.ck + dz;
+        if (!inside && (x < 0 || x >= F.n || y < 0 || y >= F.n || z < 0 || z >= F.n)) continue;
+        const long a = x + y * n1 + z * n12;
+        const float d = F.d[a], dT = F.dT[a], ex = F.ex[a], ey = F.ey[a], ez = F.ez[a];
+        const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+        acc[P_F] += d;
+        acc[P_N] += 1.0f;
+        acc[P_G + 0] += d * fx; acc[P_G + 1] += d * fy; acc[P_G + 2] += d * fz;
+        acc[P_GF + 0] += dT * fx; acc[P_GF + 1] += dT * fy; acc[P_GF + 2] += dT * fz;
+        acc[P_E + 0] += 2.0f * (ex * fx);
+        acc[P_E + 1] += 2.0f * (ey * fy);
+        acc[P_E + 2] += 2.0f * (ez * fz);
+        acc[P_E + 3] += ex * fy + ey * fx;
+        acc[P_E + 4] += ex * fz + ez * fx;
+        acc[P_E + 5] += ey * fz + ez * fy;
+        if constexpr (FULL) {
+            acc[P_S + 0] += ex; acc[P_S + 1] += ey; acc[P_S + 2] += ez;
+            if (F.e2x) { acc[P_S2 + 0] += F.e2x[a]; acc[P_S2 + 1] += F.e2y[a]; acc[P_S2 + 2] += F.e2z[a]; }
+            if (F.lap) acc[P_L] += F.lap[a];
+        }
+    }
+}
+
+__device__ __forceinline__ void store_record(float *r, const float v[PR], int nf4)
+{
+    float4 *o = (float4 *)r;
+    for (int q = 0; q < nf4; q++) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+}
+
+__device__ __forceinline__ void put_fbar(float v[PR], double fb)
+{
+    float w[2];
+    __builtin_memcpy(w, &fb, sizeof fb);
+    v[P_FB] = w[0];
+    v[P_FB + 1] = w[1];
+}
+
+// Centre record (shell 1): count 1, F = delta(centre), Fbar(1) = F; FULL
+// adds S, S2 of the centre cell (lapd of shell 1 is not used by d2F, which
+// starts at cell 2).  Returns F.
+template <bool FULL>
+__device__ __forceinline__ float center_values(const GatherFields &F, const PeakParam &pk, float v[PR])
+{
+    const long c = pk.ci + pk.cj * (long)F.n + pk.ck * (long)F.n * F.n;
+#pragma unroll
+    for (int f = 0; f < PR; f++) v[f] = 0.0f;
+    v[P_F] = F.d[c];
+    v[P_N] = 1.0f;
+    if constexpr (FULL) {
+        v[P_S + 0] = F.ex[c]; v[P_S + 1] = F.ey[c]; v[P_S + 2] = F.ez[c];
+        if (F.e2x) { v[P_S2 + 0] = F.e2x[c]; v[P_S2 + 1] = F.e2y[c]; v[P_S2 + 2] = F.e2z[c]; }
+    }
+    put_fbar(v, (double)v[P_F]);
+    return v[P_F];
+}
+
+// Inclusive prefix sum over the 64 lanes (fixed pattern, deterministic).
+__device__ __forceinline__ double wave_scan(double x)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    return x;
+}
+
+// ------------------------------------------------------- small peaks
+// One segment of small peaks with equal m, in sorted order [lo, hi); its
+// blocks of P peaks are numbered from blk0.
+struct SmallSeg {
+    int32_t m, G, P, doff;  // shells, groups, peaks per wave, first group descriptor
+    int32_t soff, pad;      // first shell boundary (groups of shell s: [segs[soff+s-2], segs[soff+s-1]))
+    int64_t lo, hi, blk0;
+};
+
+struct SmallArgs {
+    GatherFields F;
+    const PeakParam *pps;
+    const GroupDesc *desc;   // groups of shells 2..m, sorted by shell
+    const int32_t *segs;
+    const SmallSeg *seg;
+    int nseg;
+    int64_t nblk;
+    float *prof;
+    const int64_t *prof_off;
+    const double *rad;
+};
+
+// P * m <= 64 for every small m (P = floor(64 / groups)), so one pass of the
+// 64 lanes writes all records of a block.
+__global__ void __launch_bounds__(64) k_small(SmallArgs A)
+{
+    __shared__ float part[PR][64];
+    __shared__ double sdF[64], sfb[64];
+    const int lane = threadIdx.x;
+    for (int64_t blk = blockIdx.x; blk < A.nblk; blk += gridDim.x) {
+        int si = 0;
+        while (si + 1 < A.nseg && blk >= A.seg[si + 1].blk0) si++;
+        const SmallSeg S = A.seg[si];
+        const int m = S.m, G = S.G, P = S.P;
+        const int64_t pbase = S.lo + (blk - S.blk0) * P;
+        float acc[PR];
+#pragma unroll
+        for (int f = 0; f < PR; f++) acc[f] = 0.0f;
+        if (G > 0) {
+            const int q = lane / G, g = lane - q * G;
+            const int64_t idx = pbase + q;
+            if (q < P && idx < S.hi) group_sum<true>(A.F, A.pps[idx], A.desc[S.doff + g], acc);
+        }
+#pragma unroll
+        for (int f = 0; f < PR; f++) part[f][lane] = acc[f];
+        __syncthreads();
+        // record of (peak q, shell s) on lane o = q m + s - 1
+        const int q = lane / m, s = 1 + (lane - q * m);
+        const int64_t idx = pbase + q;
+        const bool act = lane < P * m && idx < S.hi;
+        float v[PR];
+#pragma unroll
+        for (int f = 0; f < PR; f++) v[f] = 0.0f;
+        if (act) {
+            if (s == 1) {
+                center_values<true>(A.F, A.pps[idx], v);
+            } else {
+                const int g0 = q * G + A.segs[S.soff + s - 2], g1 = q * G + A.segs[S.soff + s - 1];
+                for (int gg = g0; gg < g1; gg++) {
+#pragma unroll
+                    for (int f = 0; f < PR; f++) v[f] += part[f][gg];
+                }
+            }
+        }
+        sdF[lane] = v[P_N] > 0.0f ? (double)v[P_F] / (double)v[P_N] : 0.0;
+        __syncthreads();
+        // Fbar recursion (RadialShell.jl:383-390), one lane per peak
+        if (lane < P && pbase + lane < S.hi) {
+            const double *dF = sdF + lane * m;
+            double fb = dF[0];  // F of the centre (count 1)
+            sfb[lane * m] = fb;
+            for (int t = 2; t <= m; t++) {
+                const double rp = A.rad[t - 2], rs = A.rad[t - 1];
+                const double r3p = rp * rp * rp, r3 = rs * rs * rs;
+                fb = (r3p * fb + 0.5 * (dF[t - 2] + dF[t - 1]) * (r3 - r3p)) / r3;
+                sfb[lane * m + t - 1] = fb;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            put_fbar(v, sfb[lane]);
+            store_record(A.prof + (A.prof_off[idx] + (s - 1)) * PR, v, PR / 4);
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------- wave gather
 struct GatherArgs {
     GatherFields F;
     const PeakParam *pps;
     int64_t lo, hi;
     const GroupDesc *desc;     // all batch tables
-    const int64_t *tab_off;    // per m: first descriptor of its batch table
-    const int32_t *tab_nb;     // per m: number of 64-slot batches
+    const int64_t *full_off;   // per chunk c: first descriptor of the full-chunk table
+    const int32_t *full_nb;    //   and its number of 64-slot batches
+    const int64_t *part_off;   // per m: the table of the last, partial chunk
+    const int32_t *part_nb;
     float *prof;
     const int64_t *prof_off;   // per sorted index: first shell record
+    const double *rad;
     int *queue;
-    int mcap;
+    int cs;                    // shells per chunk
 };
 
 __device__ __forceinline__ void neumaier(float *hi, float *lo, float x)
@@ -244,14 +316,19 @@ __device__ __forceinline__ void neumaier(float *hi, float *lo, float x)
     *lo += e;
 }
 
+// One wave per peak.  Per batch: each lane sums its group's images, the 14
+// partial sums go to an LDS staging row, and the first lane of each run of
+// equal shells (the host sorted every batch by shell) adds its run in lane
+// order and deposits it into the chunk's bins.  The records of a chunk are
+// written with Fbar, whose recursion is carried as the prefix sum
+// Q(s) = Fbar(s) r^3(s) = sum_{t=2..s} (dF(t-1) + dF(t)) / 2 (r^3(t) - r^3(t-1)).
 __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
 {
     extern __shared__ float lds[];
     const int lane = threadIdx.x;
-    const int mcap = A.mcap;
-    float *bh = lds, *bl = lds + P_NB * mcap;
-    const GatherFields &F = A.F;
-    const long n1 = F.n, n12 = (long)F.n * F.n;
+    const int cs = A.cs;
+    float *stg = lds;                  // [P_NB][64]
+    float *bh = lds + P_NB * 64, *bl = bh + P_NB * cs;
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(A.queue, 1);
@@ -260,118 +337,131 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
         if (idx >= A.hi) break;
         const PeakParam pk = A.pps[idx];
         const int m = pk.m;
-        const bool inside = pk.flags & 1;
-        for (int f = 0; f < P_NB; f++)
-            for (int s2 = lane; s2 < m - 1; s2 += 64) { bh[f * mcap + s2] = 0.0f; bl[f * mcap + s2] = 0.0f; }
-        __syncthreads();
-        const GroupDesc *tab = A.desc + A.tab_off[m];
-        const int nb = A.tab_nb[m];
-        for (int b = 0; b < nb; b++) {
-            const GroupDesc g = tab[b * 64 + lane];
-            float acc[P_NB];
-#pragma unroll
-            for (int f = 0; f < P_NB; f++) acc[f] = 0.0f;
-            if (g.s >= 2) {
-                const int l = g.lmn & 255, dj = (g.lmn >> 8) & 255, dk = (g.lmn >> 16) & 255;
-#pragma unroll
-                for (int img = 0; img < 8; img++) {
-                    const int sx = (img & 1) ? -1 : 1, sy = (img & 2) ? -1 : 1, sz = (img & 4) ? -1 : 1;
-                    if ((sx < 0 && l == 0) || (sy < 0 && dj == 0) || (sz < 0 && dk == 0)) continue;
-                    const int dx = sx * l, dy = sy * dj, dz = sz * dk;
-                    const int x = pk.ci + dx, y = pk.cj + dy, z = pk.ck + dz;
-                    if (!inside && (x < 0 || x >= F.n || y < 0 || y >= F.n || z < 0 || z >= F.n)) continue;
-                    const long a = x + y * n1 + z * n12;
-                    const float d = F.d[a], dT = F.dT[a], ex = F.ex[a], ey = F.ey[a], ez = F.ez[a];
-                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-                    acc[P_F] += d;
-                    acc[P_N] += 1.0f;
-                    acc[P_G + 0] += d * fx; acc[P_G + 1] += d * fy; acc[P_G + 2] += d * fz;
-                    acc[P_GF + 0] += dT * fx; acc[P_GF + 1] += dT * fy; acc[P_GF + 2] += dT * fz;
-                    acc[P_E + 0] += 2.0f * (ex * fx);
-                    acc[P_E + 1] += 2.0f * (ey * fy);
-                    acc[P_E + 2] += 2.0f * (ez * fz);
-                    acc[P_E + 3] += ex * fy + ey * fx;
-                    acc[P_E + 4] += ex * fz + ez * fx;
-                    acc[P_E + 5] += ey * fz + ez * fy;
-                }
-                const int s2 = g.s - 2;
-#pragma unroll
-                for (int f = 0; f < P_NB; f++) neumaier(&bh[f * mcap + s2], &bl[f * mcap + s2], acc[f]);
-            }
-        }
-        __syncthreads();
         float *rec = A.prof + A.prof_off[idx] * PR;
-        for (int s = 1 + lane; s <= m; s += 64) {
-            float *r = rec + (int64_t)(s - 1) * PR;
-            if (s == 1) {
-                r[P_F] = F.d[pk.ci + pk.cj * n1 + pk.ck * n12];
-                r[P_N] = 1.0f;
-                for (int f = 2; f < P_NB; f++) r[f] = 0.0f;
-            } else {
-                for (int f = 0; f < P_NB; f++) r[f] = bh[f * mcap + s - 2] + bl[f * mcap + s - 2];
-            }
+        float F1;
+        {
+            float v[PR];
+            F1 = center_values<false>(A.F, pk, v);
+            if (lane == 0) store_record(rec, v, 4);
         }
-        __syncthreads();
+        double carry = 0.0, dFprev = (double)F1;
+        const int nch = m >= 2 ? (m - 2) / cs + 1 : 0;
+        for (int c = 0; c < nch; c++) {
+            const int s0 = 2 + c * cs, s1 = min(m, s0 + cs - 1), ns = s1 - s0 + 1;
+            for (int f = 0; f < P_NB; f++)
+                for (int b = lane; b < ns; b += 64) { bh[f * cs + b] = 0.0f; bl[f * cs + b] = 0.0f; }
+            __syncthreads();
+            const bool full = s0 + cs - 1 <= m;
+            const GroupDesc *tab = A.desc + (full ? A.full_off[c] : A.part_off[m]);
+            const int nb = full ? A.full_nb[c] : A.part_nb[m];
+            for (int b = 0; b < nb; b++) {
+                const GroupDesc g = tab[b * 64 + lane];
+                float acc[PR];
+#pragma unroll
+                for (int f = 0; f < PR; f++) acc[f] = 0.0f;
+                if (g.s >= 2) group_sum<false>(A.F, pk, g, acc);
+#pragma unroll
+                for (int f = 0; f < P_NB; f++) stg[f * 64 + lane] = acc[f];
+                __syncthreads();
+                const int len = (g.lmn >> 24) & 127;
+                if (len > 0) {
+                    float v[P_NB];
+#pragma unroll
+                    for (int f = 0; f < P_NB; f++) v[f] = stg[f * 64 + lane];
+                    for (int t = 1; t < len; t++) {
+#pragma unroll
+                        for (int f = 0; f < P_NB; f++) v[f] += stg[f * 64 + lane + t];
+                    }
+                    const int sb = g.s - s0;
+#pragma unroll
+                    for (int f = 0; f < P_NB; f++) neumaier(&bh[f * cs + sb], &bl[f * cs + sb], v[f]);
+                }
+                __syncthreads();
+            }
+            for (int b0 = 0; b0 < ns; b0 += 64) {
+                const int b = b0 + lane;
+                const bool valid = b < ns;
+                const int s = s0 + b;
+                float v[PR];
+#pragma unroll
+                for (int f = 0; f < PR; f++) v[f] = 0.0f;
+                double term = 0.0, dF = 0.0, r3 = 1.0;
+                if (valid) {
+#pragma unroll
+                    for (int f = 0; f < P_NB; f++) v[f] = bh[f * cs + b] + bl[f * cs + b];
+                    dF = v[P_N] > 0.0f ? (double)v[P_F] / (double)v[P_N] : 0.0;
+                }
+                double dFm = __shfl_up(dF, 1, 64);
+                if (lane == 0) dFm = dFprev;
+                if (valid) {
+                    const double rp = A.rad[s - 2], rs = A.rad[s - 1];
+                    const double r3p = rp * rp * rp;
+                    r3 = rs * rs * rs;
+                    term = 0.5 * (dFm + dF) * (r3 - r3p);
+                }
+                const double Q = carry + wave_scan(term);
+                if (valid) {
+                    put_fbar(v, Q / r3);
+                    store_record(rec + (int64_t)(s - 1) * PR, v, 4);
+                }
+                const int last = min(63, ns - 1 - b0);
+                carry = __shfl(Q, last, 64);
+                dFprev = __shfl(dF, last, 64);
+            }
+            __syncthreads();
+        }
     }
 }
 
-// ------------------------------------------------------------ lane post
+// --------------------------------------------------------------- post
+template <bool FULL>
 struct AosProfile {
     float *rec;
     const double *radt;
+    bool lapd;
+    __device__ const float *r(int s) const { return rec + (int64_t)(s - 1) * PR; }
     __device__ double rad(int s) const { return radt[s - 1]; }
-    __device__ double Fbar(int s) const { return *(const double *)(rec + (int64_t)(s - 1) * PR + P_FB); }
-    __device__ double count(int s) const { return (double)rec[(int64_t)(s - 1) * PR + P_N]; }
-    __device__ double S(int, int) const { return 0.0; }
-    __device__ double S2(int, int) const { return 0.0; }
-    __device__ double lap(int) const { return 0.0; }
-    __device__ bool has_lapd() const { return false; }
-    __device__ void strain_inputs(int s, double G[3], double Gf[3], double Es[6]) const
+    __device__ double Fbar(int s) const { return *(const double *)(r(s) + P_FB); }
+    __device__ double count(int s) const { return (double)r(s)[P_N]; }
+    __device__ double S(int s, int q) const { return FULL ? (double)r(s)[P_S + q] : 0.0; }
+    __device__ double S2(int s, int q) const { return FULL ? (double)r(s)[P_S2 + q] : 0.0; }
+    __device__ double lap(int s) const { return FULL ? (double)r(s)[P_L] : 0.0; }
+    __device__ bool has_lapd() const { return FULL && lapd; }
+    __device__ void strain_raw(int s, float G[3], float Gf[3], float Es[6]) const
     {
-        if (s == 1) {
-            for (int i = 0; i < 3; i++) G[i] = Gf[i] = 0.0;
-            for (int i = 0; i < 6; i++) Es[i] = 0.0;
-            return;
-        }
-        const float *r = rec + (int64_t)(s - 1) * PR;
-        const double inv = 1.0 / radt[s - 1];
-        const double rr = radt[s - 1];
-        (void)inv;
-        for (int i = 0; i < 3; i++) {
-            G[i] = (double)r[P_G + i] / rr;
-            Gf[i] = (double)r[P_GF + i] / rr;
-        }
-        for (int i = 0; i < 6; i++) Es[i] = (double)r[P_E + i] / rr;
+        const float4 *q = (const float4 *)r(s);
+        const float4 a = q[0], b = q[1], c = q[2];
+        G[0] = a.x; G[1] = a.y; G[2] = a.z;
+        Gf[0] = a.w; Gf[1] = b.x; Gf[2] = b.y;
+        Es[0] = b.z; Es[1] = b.w; Es[2] = c.x; Es[3] = c.y; Es[4] = c.z; Es[5] = c.w;
     }
 };
 
+template <bool FULL>
 __global__ void __launch_bounds__(64) k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
                                                   float *__restrict__ prof, const int64_t *__restrict__ prof_off,
-                                                  PostConst C, ResultView R)
+                                                  PostConst C, ResultView R, int has_lap)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += stride) {
         const PeakParam pk = pps[idx];
-        const int m = pk.m;
         float *rec = prof + prof_off[idx] * PR;
-        {   // Fbar recursion (RadialShell.jl:383-390) into slots 14-15
-            double fb = (double)rec[P_F];
-            *(double *)(rec + P_FB) = fb;
-            double dFbarp = fb;
-            for (int s = 2; s <= m; s++) {
-                const float *r = rec + (int64_t)(s - 1) * PR;
-                const double nn = (double)r[P_N];
-                const double dF = nn > 0.0 ? (double)r[P_F] / nn : 0.0;
-                const double rp = C.rad[s - 2], rs = C.rad[s - 1];
-                const double r3p = rp * rp * rp, r3 = rs * rs * rs;
-                fb = (r3p * fb + 0.5 * (dFbarp + dF) * (r3 - r3p)) / r3;
-                *(double *)(rec + (int64_t)(s - 1) * PR + P_FB) = fb;
-                dFbarp = dF;
-            }
-        }
-        AosProfile P{rec, C.rad};
-        analyse_post<false, false>(P, pk, C, R, pk.order);
-        R.nshells[pk.order] = m;
+        AosProfile<FULL> P{rec, C.rad, has_lap != 0};
+        analyse_post<false, FULL>(P, pk, C, R, pk.order);
+        R.nshells[pk.order] = pk.m;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_post_wave(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
+                                                  float *__restrict__ prof, const int64_t *__restrict__ prof_off,
+                                                  PostConst C, ResultView R)
+{
+    for (int64_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
+        const PeakParam pk = pps[idx];
+        float *rec = prof + prof_off[idx] * PR;
+        AosProfile<false> P{rec, C.rad, false};
+        analyse_post<true, false>(P, pk, C, R, pk.order);
+        if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
     }
 }
 
@@ -390,47 +480,22 @@ __global__ void __launch_bounds__(64) k_finish(GatherFields F, const PeakParam *
 }
 
 // ------------------------------------------------------------- host side
-// Octant groups of shells 2..m in row-major order packed into 64-slot batches
-// with no shell repeated inside a batch (greedy over a window of open
-// batches, which keeps row-major locality).
-static void pack_groups(const ShellTable &T, int m, std::vector<GroupDesc> &out, int &nbatch)
-{
-    const int r2m = T.shell_r2[m - 1];
-    const int R = (int)std::floor(std::sqrt((double)r2m));
-    std::vector<int32_t> shell_of_r2((size_t)r2m + 1, 0);
-    for (int s = 1; s <= m; s++) shell_of_r2[T.shell_r2[s - 1]] = s;
-    struct B { std::vector<GroupDesc> g; std::vector<int32_t> shells; };
-    std::vector<B> bs;
-    const size_t K = 8;  // open batches considered
-    for (int dk = 0; dk <= R; dk++)
-        for (int dj = 0; dj <= R; dj++)
-            for (int l = 0; l <= R; l++) {
-                const int r2 = l * l + dj * dj + dk * dk;
-                if (r2 == 0 || r2 > r2m) continue;
-                const int s = shell_of_r2[r2];
-                GroupDesc g{l | (dj << 8) | (dk << 16), s};
-                bool placed = false;
-                for (size_t b = bs.size() > K ? bs.size() - K : 0; b < bs.size() && !placed; b++) {
-                    B &bb = bs[b];
-                    if (bb.g.size() >= 64) continue;
-                    if (std::find(bb.shells.begin(), bb.shells.end(), s) != bb.shells.end()) continue;
-                    bb.g.push_back(g);
-                    bb.shells.push_back(s);
-                    placed = true;
-                }
-                if (!placed) { bs.emplace_back(); bs.back().g.push_back(g); bs.back().shells.push_back(s); }
-            }
-    nbatch = (int)bs.size();
-    out.assign((size_t)nbatch * 64, GroupDesc{0, -1});
-    for (int b = 0; b < nbatch; b++)
-        for (size_t q = 0; q < bs[b].g.size(); q++) out[(size_t)b * 64 + q] = bs[b].g[q];
-}
+struct SmallTab {
+    int32_t doff = 0, soff = 0, G = 0;
+};
 
 struct PersistState {
-    int rmax = -1;
-    std::map<int, std::pair<int64_t, int>> tables;  // m -> (descriptor offset, nbatch)
-    std::vector<GroupDesc> desc;
-    DevBuf d_desc, d_off, d_nb;
+    int rmax = -1, cs = 0;
+    std::vector<int32_t> shell_of_r2;
+    std::vector<GroupDesc> desc;                      // chunk tables
+    std::vector<std::pair<int64_t, int>> full;        // per chunk: (offset, batches)
+    std::map<int, std::pair<int64_t, int>> part;      // per m: last partial chunk
+    std::vector<GroupDesc> sdesc;                     // small tables
+    std::vector<int32_t> ssegs;
+    std::vector<SmallTab> stab;                       // index m (0..small_max)
+    int small_max = 0;
+    std::vector<SmallSeg> segs_host;
+    DevBuf d_desc, d_full_off, d_full_nb, d_part_off, d_part_nb, d_sdesc, d_ssegs, d_seg;
     bool dirty = true;
 };
 
@@ -443,32 +508,150 @@ static PersistState &state_of(pp_ctx *ctx)
     return g_state[ctx];
 }
 
-static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms)
+void persist_release(pp_ctx *ctx)
+{
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    auto it = g_state.find(ctx);
+    if (it == g_state.end()) return;
+    PersistState &st = it->second;
+    DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb,
+                      &st.d_sdesc, &st.d_ssegs, &st.d_seg};
+    for (DevBuf *b : bufs) b->release();
+    g_state.erase(it);
+}
+
+// Octant groups (l, dj, dk >= 0, not all zero) of shells [slo, shi], in
+// row-major order (dk, dj, l).
+template <class Fn>
+static void for_groups(const ShellTable &T, const std::vector<int32_t> &shell_of_r2, int slo, int shi, Fn fn)
+{
+    const int r2lo = T.shell_r2[slo - 1], r2hi = T.shell_r2[shi - 1];
+    const int R = (int)std::floor(std::sqrt((double)r2hi));
+    for (int dk = 0; dk <= R; dk++)
+        for (int dj = 0; dj <= R; dj++) {
+            const int a = dk * dk + dj * dj;
+            if (a > r2hi) continue;
+            int l = 0;
+            if (r2lo - a > 0) {
+                l = (int)std::floor(std::sqrt((double)(r2lo - a)));
+                while (l * l + a < r2lo) l++;
+                while (l > 0 && (l - 1) * (l - 1) + a >= r2lo) l--;
+            }
+            for (; l * l + a <= r2hi; l++) {
+                const int r2 = l * l + a;
+                if (r2 == 0) continue;
+                fn(GroupDesc{l | (dj << 8) | (dk << 16), shell_of_r2[r2]});
+            }
+        }
+}
+
+// Groups of shells [slo, shi] in row-major order cut into consecutive
+// 64-slot batches (neighbouring x-runs share cache lines); inside a batch the
+// groups are sorted by shell and the first lane of each run of equal shells
+// carries the run length in lmn bits 24-30 (its deposit lane).
+static void pack_groups(const ShellTable &T, const std::vector<int32_t> &shell_of_r2, int slo, int shi,
+                        std::vector<GroupDesc> &out, int &nbatch)
+{
+    std::vector<GroupDesc> g;
+    for_groups(T, shell_of_r2, slo, shi, [&](GroupDesc d) { g.push_back(d); });
+    nbatch = (int)((g.size() + 63) / 64);
+    const size_t base = out.size();
+    out.resize(base + (size_t)nbatch * 64, GroupDesc{0, -1});
+    for (int b = 0; b < nbatch; b++) {
+        const size_t lo = (size_t)b * 64, hi = std::min(g.size(), lo + 64);
+        std::stable_sort(g.begin() + lo, g.begin() + hi,
+                         [](const GroupDesc &x, const GroupDesc &y) { return x.s < y.s; });
+        for (size_t q = lo; q < hi;) {
+            size_t r = q;
+            while (r < hi && g[r].s == g[q].s) r++;
+            g[q].lmn |= (int32_t)(r - q) << 24;
+            q = r;
+        }
+        for (size_t q = lo; q < hi; q++) out[base + q] = g[q];
+    }
+}
+
+// Chunk tables for the m in ms, and the small-peak tables (once per shell
+// table).
+static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
 {
     PersistState &st = state_of(ctx);
     const ShellTable &T = ctx->shells;
-    if (st.rmax != T.rmax) { st = PersistState(); st.rmax = T.rmax; }
+    if (st.rmax != T.rmax || st.cs != cs) {
+        DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb,
+                          &st.d_sdesc, &st.d_ssegs, &st.d_seg};
+        for (DevBuf *b : bufs) b->release();
+        st = PersistState();
+        st.rmax = T.rmax;
+        st.cs = cs;
+        st.shell_of_r2.assign((size_t)T.shell_r2[T.nshells - 1] + 1, 0);
+        for (int s = 1; s <= T.nshells; s++) st.shell_of_r2[T.shell_r2[s - 1]] = s;
+        // small tables: groups of shells 2..m sorted by shell
+        st.stab.assign(SMALL_MAX + 1, SmallTab());
+        st.small_max = 1;
+        for (int m = 2; m <= std::min(SMALL_MAX, (int)T.nshells); m++) {
+            std::vector<GroupDesc> g;
+            for_groups(T, st.shell_of_r2, 2, m, [&](GroupDesc d) { g.push_back(d); });
+            if (g.size() > 64) break;
+            std::stable_sort(g.begin(), g.end(), [](const GroupDesc &a, const GroupDesc &b) { return a.s < b.s; });
+            SmallTab t;
+            t.doff = (int32_t)st.sdesc.size();
+            t.soff = (int32_t)st.ssegs.size();
+            t.G = (int32_t)g.size();
+            size_t q = 0;
+            for (int s = 2; s <= m + 1; s++) {
+                while (q < g.size() && g[q].s < s) q++;
+                st.ssegs.push_back((int32_t)q);
+            }
+            st.sdesc.insert(st.sdesc.end(), g.begin(), g.end());
+            st.stab[m] = t;
+            st.small_max = m;
+        }
+        PP_CHECK(st.d_sdesc.grow(std::max<size_t>(1, st.sdesc.size()) * sizeof(GroupDesc)));
+        PP_CHECK(st.d_ssegs.grow(std::max<size_t>(1, st.ssegs.size()) * sizeof(int32_t)));
+        if (!st.sdesc.empty())
+            PP_HIP(hipMemcpy(st.d_sdesc.ptr, st.sdesc.data(), st.sdesc.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
+        if (!st.ssegs.empty())
+            PP_HIP(hipMemcpy(st.d_ssegs.ptr, st.ssegs.data(), st.ssegs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     for (int m : ms) {
-        if (m < 2 || st.tables.count(m)) continue;
-        std::vector<GroupDesc> d;
-        int nb = 0;
-        pack_groups(T, m, d, nb);
-        st.tables[m] = {(int64_t)st.desc.size(), nb};
-        st.desc.insert(st.desc.end(), d.begin(), d.end());
-        st.dirty = true;
+        if (m < 2) continue;
+        const int nfull = (m - 1) / cs;  // chunks entirely inside [2, m]
+        while ((int)st.full.size() < nfull) {
+            const int c = (int)st.full.size();
+            int nb = 0;
+            const int64_t off = (int64_t)st.desc.size();
+            pack_groups(T, st.shell_of_r2, 2 + c * cs, 1 + (c + 1) * cs, st.desc, nb);
+            st.full.push_back({off, nb});
+            st.dirty = true;
+        }
+        if ((m - 1) % cs != 0 && !st.part.count(m)) {
+            const int c = (m - 2) / cs;
+            int nb = 0;
+            const int64_t off = (int64_t)st.desc.size();
+            pack_groups(T, st.shell_of_r2, 2 + c * cs, m, st.desc, nb);
+            st.part[m] = {off, nb};
+            st.dirty = true;
+        }
     }
     if (st.dirty) {
         const int ns = T.nshells + 1;
-        std::vector<int64_t> off(ns, 0);
-        std::vector<int32_t> nb(ns, 0);
-        for (auto &kv : st.tables) { off[kv.first] = kv.second.first; nb[kv.first] = kv.second.second; }
+        const size_t nf = std::max<size_t>(1, st.full.size());
+        std::vector<int64_t> foff(nf, 0), poff(ns, 0);
+        std::vector<int32_t> fnb(nf, 0), pnb(ns, 0);
+        for (size_t c = 0; c < st.full.size(); c++) { foff[c] = st.full[c].first; fnb[c] = st.full[c].second; }
+        for (auto &kv : st.part) { poff[kv.first] = kv.second.first; pnb[kv.first] = kv.second.second; }
         PP_CHECK(st.d_desc.grow(std::max<size_t>(1, st.desc.size()) * sizeof(GroupDesc)));
-        PP_CHECK(st.d_off.grow(ns * sizeof(int64_t)));
-        PP_CHECK(st.d_nb.grow(ns * sizeof(int32_t)));
+        PP_CHECK(st.d_full_off.grow(nf * sizeof(int64_t)));
+        PP_CHECK(st.d_full_nb.grow(nf * sizeof(int32_t)));
+        PP_CHECK(st.d_part_off.grow(ns * sizeof(int64_t)));
+        PP_CHECK(st.d_part_nb.grow(ns * sizeof(int32_t)));
         if (!st.desc.empty())
             PP_HIP(hipMemcpy(st.d_desc.ptr, st.desc.data(), st.desc.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
-        PP_HIP(hipMemcpy(st.d_off.ptr, off.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
-        PP_HIP(hipMemcpy(st.d_nb.ptr, nb.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(st.d_full_off.ptr, foff.data(), nf * sizeof(int64_t), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(st.d_full_nb.ptr, fnb.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(st.d_part_off.ptr, poff.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(st.d_part_nb.ptr, pnb.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
         st.dirty = false;
     }
     return PP_OK;
@@ -522,15 +705,16 @@ static int order_peaks(pp_ctx *ctx, ShellRun &S, PeakParam *&pps, int32_t *&m_so
     return PP_OK;
 }
 
-static int lds_floats_per_shell() { return 2 * P_NB; }
-
-// The persistent pipeline on an ordered batch.  export_prof keeps the profile
-// scratch for pp_shell_profiles (all peaks then go through k_gather).
+// The persistent pipeline on an ordered batch.  export_all keeps the profile
+// for pp_shell_profiles (every peak then goes through k_gather).
 static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, bool export_all,
                                std::vector<int32_t> *cnt, std::vector<double> *fbar, std::vector<int64_t> *offs)
 {
     const int64_t np = S.np;
     const int n = G.n;
+    if (np > INT_MAX) { set_error("persistent form: at most 2^31-1 peaks per call"); return PP_ERR_CAPACITY; }
+    for (double &t : ctx->timing.stage_ms) t = 0.0;
+    PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     // 1. transposed delta
     PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float)));
     {
@@ -545,117 +729,134 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     std::vector<int> hist;
     PP_CHECK(order_peaks(ctx, S, pps, m_sorted, hist));
     const int nb = (int)hist.size();
-    // classes: tiny (m <= tiny_m, lane per peak); LDS classes (m in (b/2, b]
-    // for power-of-two b, one persistent launch each, largest first); m above
-    // the LDS capacity: plain kernel.
-    const int tiny_m = export_all ? 0 : (ctx->tiny_m > 0 ? ctx->tiny_m : 16);
-    const size_t lds_max = std::min<size_t>(ctx->lds_per_block, 160 * 1024);
-    const int mcap_max = (int)(lds_max / (lds_floats_per_shell() * sizeof(float)));
     std::vector<int64_t> above(nb + 1, 0);  // above[x] = #peaks with m >= x
     for (int x = nb - 1; x >= 0; x--) above[x] = above[x + 1] + hist[x];
     auto cnt_gt = [&](int x) -> int64_t { return x < 0 ? np : above[std::min(x + 1, nb)]; };
-    const int64_t big_end = cnt_gt(mcap_max);   // sorted [0, big_end): beyond the LDS bins
-    const int64_t tiny_lo = cnt_gt(tiny_m);     // sorted [tiny_lo, np): tiny
+    const int cs = std::max(8, std::min(ctx->chunk_shells > 0 ? ctx->chunk_shells : 64, 1024));
     std::vector<int> ms;
-    int mtop = 0;
     for (int x = 2; x < nb; x++)
-        if (hist[x] && x > tiny_m && x <= mcap_max) { ms.push_back(x); mtop = x; }
-    PP_CHECK(ensure_tables(ctx, ms));
+        if (hist[x]) ms.push_back(x);
+    PP_CHECK(ensure_tables(ctx, ms, cs));
     PersistState &st = state_of(ctx);
-    // profile offsets over the gathered range [big_end, tiny_lo): exclusive
-    // scan of m (int64)
+    // classes in sorted order: [0, small_lo) wave gather, [small_lo, np) small
+    // (m <= small_m); posts: [0, coop_end) one wave per peak, the rest one
+    // lane per peak
+    const int small_m = export_all ? 0 : std::min(std::max(ctx->tiny_m, 0), st.small_max);
+    const int64_t small_lo = cnt_gt(small_m);
+    const int64_t coop_end = std::min(small_lo, cnt_gt(std::max(ctx->coop_m, 1)));
+    // 3. profile offsets: exclusive scan of m over all peaks
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const int64_t ng = tiny_lo - big_end;
     int64_t tot = 0;
-    PP_CHECK(ctx->d_scan.grow(al(std::max<int64_t>(ng, 1) * 8) * 2 + 4096));
+    PP_CHECK(ctx->d_scan.grow(al(std::max<int64_t>(np, 1) * 8) * 2 + 4096));
     int64_t *d_m64 = ctx->d_scan.as<int64_t>();
-    int64_t *d_off = (int64_t *)(ctx->d_scan.as<char>() + al(std::max<int64_t>(ng, 1) * 8));
-    if (ng > 0) {
-        hipLaunchKernelGGL(k_m_to_i64, dim3(grid1d(ctx, ng)), dim3(256), 0, ctx->stream,
-                           (const int32_t *)(m_sorted + big_end), ng, d_m64);
+    int64_t *d_off = (int64_t *)(ctx->d_scan.as<char>() + al(std::max<int64_t>(np, 1) * 8));
+    if (np > 0) {
+        hipLaunchKernelGGL(k_m_to_i64, dim3(grid1d(ctx, np)), dim3(256), 0, ctx->stream, (const int32_t *)m_sorted, np,
+                           d_m64);
         size_t tmp = 0;
-        PP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_m64, d_off, (int)ng, ctx->stream));
+        PP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_m64, d_off, (int)np, ctx->stream));
         PP_CHECK(ctx->d_cub.grow(tmp));
-        PP_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_cub.ptr, tmp, d_m64, d_off, (int)ng, ctx->stream));
+        PP_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_cub.ptr, tmp, d_m64, d_off, (int)np, ctx->stream));
         int64_t last[2];
-        PP_HIP(hipMemcpyAsync(&last[0], d_off + ng - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-        PP_HIP(hipMemcpyAsync(&last[1], d_m64 + ng - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipMemcpyAsync(&last[0], d_off + np - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipMemcpyAsync(&last[1], d_m64 + np - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));
         tot = last[0] + last[1];
     }
-    // (ctx->d_prof belongs to the plain kernel, which step 3 may run)
     PP_CHECK(ctx->d_res.grow((size_t)std::max<int64_t>(tot, 1) * PR * sizeof(float) + 4096));
     float *d_rec = ctx->d_res.as<float>();
     int *d_q = (int *)(ctx->d_misc.as<char>() + 1024);
-    PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    // 3. peaks beyond the LDS cap: plain kernel on their ordered params
-    if (big_end > 0) {
-        ShellRun S2 = S;
-        S2.np = big_end;
-        S2.pp = pps;
-        PP_CHECK(run_plain(ctx, G, S2, C, nullptr));
+    // small-peak segments (equal m, descending)
+    st.segs_host.clear();
+    int64_t nblk = 0;
+    for (int m = small_m; m >= 1; m--) {
+        const int64_t lo = cnt_gt(m), hi = cnt_gt(m - 1);
+        if (hi <= lo) continue;
+        SmallSeg sg{};
+        sg.m = m;
+        sg.G = st.stab[m].G;
+        sg.P = std::min(sg.G > 0 ? 64 / sg.G : 64, 64 / m);  // k_small writes the P * m records in one pass
+        sg.doff = st.stab[m].doff;
+        sg.soff = st.stab[m].soff;
+        sg.lo = lo;
+        sg.hi = hi;
+        sg.blk0 = nblk;
+        nblk += (hi - lo + sg.P - 1) / sg.P;
+        st.segs_host.push_back(sg);
     }
-    // 4. persistent gather per LDS class (descending m)
-    if (ng > 0) {
-        int bcap = 32;
-        while (bcap < mtop) bcap <<= 1;
-        int cls = 0;
-        for (int b = std::min(bcap, mcap_max);; ) {
-            const int blow = std::max(tiny_m, b / 2);
-            const int64_t lo = std::max(big_end, cnt_gt(b)), hi = std::min(tiny_lo, cnt_gt(blow));
-            if (hi > lo) {
-                const size_t lds = (size_t)b * lds_floats_per_shell() * sizeof(float);
-                const int waves_cu = std::max(1, std::min<int>(ctx->gather_waves_cu > 0 ? ctx->gather_waves_cu : 16,
-                                                               (int)(lds_max / lds)));
-                const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(hi - lo, (int64_t)ctx->num_cu * waves_cu));
-                hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, d_q + cls, (int64_t)1, 0);
-                GatherArgs A;
-                A.F = G; A.pps = pps; A.lo = lo; A.hi = hi;
-                A.desc = st.d_desc.as<GroupDesc>(); A.tab_off = st.d_off.as<int64_t>(); A.tab_nb = st.d_nb.as<int32_t>();
-                A.prof = d_rec; A.prof_off = d_off - big_end; A.queue = d_q + cls; A.mcap = b;
-                if (lds > 64 * 1024)
-                    (void)hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-                hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
-                PP_HIP(hipGetLastError());
-                cls++;
-            }
-            if (blow <= tiny_m || cls >= 200) break;
-            b = blow;
-        }
+    if (!st.segs_host.empty()) {
+        PP_CHECK(st.d_seg.grow(st.segs_host.size() * sizeof(SmallSeg)));
+        PP_HIP(hipMemcpyAsync(st.d_seg.ptr, st.segs_host.data(), st.segs_host.size() * sizeof(SmallSeg),
+                              hipMemcpyHostToDevice, ctx->stream));
     }
     PP_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
-    // 5. tiny peaks, one lane each
-    if (tiny_lo < np) {
-        const int TM = std::max(tiny_m, 2);
-        const int64_t lanes = std::min<int64_t>(((np - tiny_lo) + 63) / 64 * 64, (int64_t)ctx->num_cu * 64 * 16);
-        PP_CHECK(ctx->d_k2.grow((size_t)lanes * TM * (PF_NFIELD * 4 + 8) + 4096));
-        float *scr = ctx->d_k2.as<float>();
-        double *fbs = (double *)(ctx->d_k2.as<char>() + al((size_t)lanes * TM * PF_NFIELD * 4));
-        hipLaunchKernelGGL(k_tiny, dim3((unsigned)(lanes / 64)), dim3(64), 0, ctx->stream, G, (const PeakParam *)pps,
-                           tiny_lo, np, scr, fbs, TM, C, S.R);
+    // 4. gathers
+    if (small_lo > 0) {
+        const size_t lds = ((size_t)cs * 2 + 64) * P_NB * sizeof(float);
+        const size_t lds_max = std::min<size_t>(ctx->lds_per_block, 160 * 1024);
+        const int waves_cu = std::max(1, std::min<int>(ctx->gather_waves_cu > 0 ? ctx->gather_waves_cu : 16,
+                                                       (int)(lds_max / lds)));
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * waves_cu));
+        hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, d_q, (int64_t)1, 0);
+        GatherArgs A;
+        A.F = G; A.pps = pps; A.lo = 0; A.hi = small_lo;
+        A.desc = st.d_desc.as<GroupDesc>();
+        A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
+        A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
+        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs;
+        if (lds > 64 * 1024)
+            (void)hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
         PP_HIP(hipGetLastError());
     }
-    // 6. lane post + 7. finish over the gathered range
-    if (tiny_lo > big_end) {
-        const int64_t cnt_g = tiny_lo - big_end;
-        const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((cnt_g + 63) / 64, (int64_t)ctx->num_cu * 16));
-        hipLaunchKernelGGL(k_post_lane, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, big_end, tiny_lo,
-                           d_rec, (const int64_t *)(d_off - big_end), C, S.R);
+    PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    if (nblk > 0) {
+        SmallArgs A;
+        A.F = G; A.pps = pps; A.desc = st.d_sdesc.as<GroupDesc>(); A.segs = st.d_ssegs.as<int32_t>();
+        A.seg = st.d_seg.as<SmallSeg>(); A.nseg = (int)st.segs_host.size(); A.nblk = nblk;
+        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
+        hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->stream, A);
         PP_HIP(hipGetLastError());
-        const int gf = (int)std::max<int64_t>(1, std::min<int64_t>(cnt_g, (int64_t)ctx->num_cu * 16));
-        hipLaunchKernelGGL(k_finish, dim3(gf), dim3(64), 0, ctx->stream, G, (const PeakParam *)pps, big_end, tiny_lo, C,
-                           S.R);
+    }
+    PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    // 5. posts
+    if (coop_end > 0) {
+        const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * 16));
+        hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps, (int64_t)0, coop_end,
+                           d_rec, (const int64_t *)d_off, C, S.R);
+        PP_HIP(hipGetLastError());
+    }
+    auto post_lane = [&](int64_t lo, int64_t hi, bool full) -> int {
+        if (hi <= lo) return PP_OK;
+        const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64, (int64_t)ctx->num_cu * 16));
+        if (full)
+            hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
+                               d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_post_lane<false>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
+                               d_rec, (const int64_t *)d_off, C, S.R, 0);
+        PP_HIP(hipGetLastError());
+        return PP_OK;
+    };
+    PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
+    PP_CHECK(post_lane(coop_end, small_lo, false));
+    PP_CHECK(post_lane(small_lo, np, true));
+    PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
+    // 6. finish for the wave-gathered peaks
+    if (small_lo > 0) {
+        const int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * 16));
+        hipLaunchKernelGGL(k_finish, dim3(gf), dim3(64), 0, ctx->stream, G, (const PeakParam *)pps, (int64_t)0, small_lo,
+                           C, S.R);
         PP_HIP(hipGetLastError());
     }
     if (cnt) {
-        // profile export (all peaks gathered): counts and Fbar per shell in the
-        // caller's order
+        // profile export: counts and Fbar per shell in the caller's order
         std::vector<float> hr((size_t)tot * PR);
         PP_HIP(hipMemcpyAsync(hr.data(), d_rec, hr.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
         std::vector<PeakParam> hpp(np);
         PP_HIP(hipMemcpyAsync(hpp.data(), pps, np * sizeof(PeakParam), hipMemcpyDeviceToHost, ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));
-        if (big_end != 0 || tiny_lo != np) { set_error("profile export: every peak must fit the LDS bins"); return PP_ERR_ARG; }
         std::vector<int64_t> start_of(np);
         int64_t t2 = 0;
         for (int64_t i = 0; i < np; i++) { start_of[hpp[i].order] = t2; t2 += hpp[i].m; }
@@ -665,21 +866,30 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         int64_t o2 = 0;
         for (int64_t i = 0; i < np; i++) {
             const int m = hpp[i].m;
-            (*offs)[hpp[i].order] = start_of[hpp[i].order];
+            const int64_t so = start_of[hpp[i].order];
+            (*offs)[hpp[i].order] = so;
             for (int s = 0; s < m; s++) {
                 const float *r = &hr[(size_t)(o2 + s) * PR];
-                (*cnt)[start_of[hpp[i].order] + s] = (int32_t)r[P_N];
+                (*cnt)[so + s] = (int32_t)r[P_N];
                 double fb;
                 memcpy(&fb, r + P_FB, 8);
-                (*fbar)[start_of[hpp[i].order] + s] = fb;
+                (*fbar)[so + s] = fb;
             }
             o2 += m;
         }
     }
-    float gms = 0;
-    PP_HIP(hipEventSynchronize(ctx->ev[3]));
-    (void)hipEventElapsedTime(&gms, ctx->ev[2], ctx->ev[3]);
-    ctx->timing.gather_ms = gms;
+    // stage times
+    PP_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
+    PP_HIP(hipEventSynchronize(ctx->ev[8]));
+    float t[6] = {0, 0, 0, 0, 0, 0};
+    (void)hipEventElapsedTime(&t[0], ctx->ev[2], ctx->ev[3]);
+    (void)hipEventElapsedTime(&t[1], ctx->ev[3], ctx->ev[4]);
+    (void)hipEventElapsedTime(&t[2], ctx->ev[4], ctx->ev[5]);
+    (void)hipEventElapsedTime(&t[3], ctx->ev[5], ctx->ev[6]);
+    (void)hipEventElapsedTime(&t[4], ctx->ev[6], ctx->ev[7]);
+    (void)hipEventElapsedTime(&t[5], ctx->ev[7], ctx->ev[8]);
+    for (int q = 0; q < 6; q++) ctx->timing.stage_ms[q] = t[q];
+    ctx->timing.gather_ms = t[1] + t[2];
     return PP_OK;
 }
 
@@ -692,15 +902,6 @@ int persistent_profiles(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, 
                         std::vector<double> &fbar, std::vector<int64_t> &off)
 {
     return run_persistent_impl(ctx, G, S, C, true, &cnt, &fbar, &off);
-}
-
-int autotune_persistent(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks, double alatt,
-                        double rmax2rs, int *choice)
-{
-    (void)ctx; (void)fields; (void)peaks; (void)alatt; (void)rmax2rs;
-    if (choice) *choice = 0;
-    set_error("autotuner not available in this build");
-    return PP_ERR_STATE;
 }
 
 int shell_groups_build(pp_ctx *) { return PP_OK; }

@@ -60,9 +60,14 @@ struct ResultView {
 
 struct PostConst {
     const double *rad;    // rad[s-1] = sqrt(r2 of shell s)
+    const double *inv_rad; // 1 / rad[s-1] (inf for the centre)
     const double *akk;    // atab4, 1101 entries
+    const int4 *wrow;     // per mp: {offset into wtab, first m1, last m1, 0}
+    const double *wtab;   // radial-kernel weight * 1/rad[m1] per (mp, m1)
     const int32_t *cell;  // packed offsets, r^2 sorted
     const int32_t *shell_start;
+    const int32_t *shell_ge; // [r2max + 2]: first shell (1-based) with r2 >= k
+    int r2max, nshells;
     Table3 tab;
     double wRnor, aRnor;
     int n1, n2, n3, nbuff;
@@ -168,25 +173,39 @@ __device__ __forceinline__ double d_interp(const Table3 &t, double x, double y, 
     return c0 * (1.0 - fz) + c1 * fz;
 }
 
-// first s in [lo, hi] (1-based) with rad(s) > x, or hi+1
-template <class PV>
-__device__ __forceinline__ int first_rad_gt(const PV &P, int lo, int hi, double x)
+// Radius searches over the shell table.  rad(s) = sqrt(r2(s)) is strictly
+// increasing and r2 is an integer, so shell_ge[k] (the first shell with
+// r2 >= k) taken at k = floor(x^2) - 1 starts at or below the answer and a
+// few exact comparisons on rad finish it; the index found is the one of the
+// reference's linear scans.
+// First s in [lo, hi] (1-based) with rad(s) > x, or hi+1.
+__device__ __forceinline__ int first_rad_gt(const PostConst &C, int lo, int hi, double x)
 {
-    while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        if (P.rad(mid) > x) hi = mid - 1; else lo = mid + 1;
+    if (lo > hi) return lo;
+    if (x != x) return hi + 1;
+    int s = 1;
+    if (x > 0.0) {
+        const double xx = x * x;
+        const int kk = xx >= (double)C.r2max ? C.r2max + 1 : (int)xx;
+        s = C.shell_ge[kk > 0 ? kk - 1 : 0];
     }
-    return lo;
+    while (s <= C.nshells && C.rad[s - 1] <= x) s++;
+    return s < lo ? lo : (s > hi ? hi + 1 : s);
 }
-// last s in [lo, hi] with rad(s) < x, or lo-1
-template <class PV>
-__device__ __forceinline__ int last_rad_lt(const PV &P, int lo, int hi, double x)
+// Last s in [lo, hi] with rad(s) < x, or lo-1.
+__device__ __forceinline__ int last_rad_lt(const PostConst &C, int lo, int hi, double x)
 {
-    while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        if (P.rad(mid) < x) lo = mid + 1; else hi = mid - 1;
+    if (lo > hi) return hi;
+    if (x != x) return lo - 1;
+    int s = 1;  // first shell with rad >= x
+    if (x > 0.0) {
+        const double xx = x * x;
+        const int kk = xx >= (double)C.r2max ? C.r2max + 1 : (int)xx;
+        s = C.shell_ge[kk > 0 ? kk - 1 : 0];
+        while (s <= C.nshells && C.rad[s - 1] < x) s++;
     }
-    return hi;
+    const int t = s - 1;
+    return t > hi ? hi : (t < lo ? lo - 1 : t);
 }
 
 struct Strain {
@@ -208,44 +227,34 @@ template <bool COOP> struct Exec {
     __device__ static bool leader() { return COOP ? (threadIdx.x & 63) == 0 : true; }
 };
 
-// kernel_strain (RadialShell.jl:228-277).
+// kernel_strain (RadialShell.jl:228-277).  The weight of shell m1 around
+// shell mp depends on the two radii only, so it comes from a table built once
+// per shell table (k_wtab); the row of mp holds exactly the shells with
+// (rad(mp) - rad(m1))^2 < 4, the reference's cut, so clamping [mlow, mupp]
+// to the row is the reference's skip.
 template <bool COOP, class PV>
 __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlow, int mupp)
 {
     using X = Exec<COOP>;
     double E[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
-    const double rmp = P.rad(mp);
-    for (int m1 = mlow + X::lane(); m1 <= mupp; m1 += X::W) {
-        double wt;
-        const double r1 = P.rad(m1);
-        if (mp > 1) {
-            double u0 = rmp - r1;
-            double u02 = u0 * u0;
-            if (!(u02 < 4.0)) continue;
-            double u12 = rmp * rmp + r1 * r1;
-            double den = u12 - u02;
-            wt = d_hRinteg(u0, u12) / (den * den);
-        } else {
-            double u0 = r1;
-            double con = 100.0 * u0 * u0;
-            int icon = (int)floor(con);
-            double diff = con - (double)icon;
-            double aww = C.akk[icon] + diff * (C.akk[icon + 1] - C.akk[icon]);
-            wt = 0.5 * aww / r1;
-        }
-        double G[3], Gf[3], Es[6];
-        P.strain_inputs(m1, G, Gf, Es);
+    const int4 row = C.wrow[mp - 1];
+    const int a = max(mlow, row.y), b = min(mupp, row.z);
+    for (int m1 = a + X::lane(); m1 <= b; m1 += X::W) {
+        const double w = C.wtab[row.x + (m1 - row.y)];
+        float G[3], Gf[3], Es[6];
+        P.strain_raw(m1, G, Gf, Es);
 #pragma unroll
         for (int L = 0; L < 3; L++) {
-            g[L] += wt * G[L];
-            gf[L] += wt * Gf[L];
+            g[L] += w * (double)G[L];
+            gf[L] += w * (double)Gf[L];
         }
         // Es = {SR00+SR00, SR11+SR11, SR22+SR22, SR01+SR10, SR02+SR20, SR12+SR21}
+        const double hw = 0.5 * w;
 #pragma unroll
-        for (int q = 0; q < 6; q++) E[q] -= 0.5 * wt * Es[q];
+        for (int q = 0; q < 6; q++) E[q] -= hw * (double)Es[q];
     }
     Strain s;
-    const double scale = mp > 1 ? C.wRnor / rmp : C.aRnor;
+    const double scale = mp > 1 ? C.wRnor / P.rad(mp) : C.aRnor;
 #pragma unroll
     for (int i = 0; i < 6; i++) s.E[i] = X::sum(E[i]) * scale;
 #pragma unroll
@@ -254,6 +263,31 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
         s.gf[L] = X::sum(gf[L]) * scale;
     }
     return s;
+}
+
+// One weight of kernel_strain: hRinteg over the pair's overlap for mp > 1,
+// the atab4 lookup at the centre (mp = 1); the shell sum's 1/rad(m1)
+// (RadialShell.jl:412-418) is folded in.
+__device__ __forceinline__ double strain_weight(const double *rad, const double *inv_rad, const double *akk, int mp,
+                                                int m1)
+{
+    const double rmp = rad[mp - 1], r1 = rad[m1 - 1];
+    double wt;
+    if (mp > 1) {
+        const double u0 = rmp - r1;
+        const double u02 = u0 * u0;
+        const double u12 = rmp * rmp + r1 * r1;
+        const double den = u12 - u02;
+        wt = d_hRinteg(u0, u12) / (den * den);
+    } else {
+        const double u0 = r1;
+        const double con = 100.0 * u0 * u0;
+        const int icon = (int)floor(con);
+        const double diff = con - (double)icon;
+        const double aww = akk[icon] + diff * (akk[icon + 1] - akk[icon]);
+        wt = 0.5 * aww / r1;
+    }
+    return wt * inv_rad[m1 - 1];
 }
 
 __device__ __forceinline__ bool finite6(const double E[6])
@@ -382,9 +416,9 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         const int mrf = pk.mrf;
         double rlow_rf = fmax(P.rad(mrf) - 2.0, 0.0);
         int mlow_rf = mrf;
-        if (mrf > 1) { int a = first_rad_gt(P, 1, mrf - 1, rlow_rf); if (a <= mrf - 1) mlow_rf = a; }
+        if (mrf > 1) { int a = first_rad_gt(C, 1, mrf - 1, rlow_rf); if (a <= mrf - 1) mlow_rf = a; }
         int mupp_rf = mrf;
-        { int b = last_rad_lt(P, mrf + 1, m, P.rad(mrf) + 2.0); if (b >= mrf + 1) mupp_rf = b; }
+        { int b = last_rad_lt(C, mrf + 1, m, P.rad(mrf) + 2.0); if (b >= mrf + 1) mupp_rf = b; }
         Strain srf = kernel_strain<COOP>(P, C, mrf, mlow_rf, mupp_rf);
         const double gradpkrf[3] = {srf.g[0], srf.g[1], srf.g[2]};
 
@@ -406,7 +440,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
             }
             if (!found) mupp = m;
             else {
-                int b = last_rad_lt(P, m0 + 1, m, P.rad(m0) + 2.0);
+                int b = last_rad_lt(C, m0 + 1, m, P.rad(m0) + 2.0);
                 mupp = b >= m0 + 1 ? b : m0;
             }
         } else {
@@ -426,14 +460,14 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 }
             }
             if (!found) { status = DUMP_NO_FCRIT; goto nocol; }
-            int b = last_rad_lt(P, m0 + 1, m, P.rad(m0) + 2.0);
+            int b = last_rad_lt(C, m0 + 1, m, P.rad(m0) + 2.0);
             mupp = b >= m0 + 1 ? b : m0;
         }
 
         // ---- Phase 5: inward walk (RadialShell.jl:540-704) ----
         double rlow = fmax(P.rad(m0) - 2.0, 0.0);
         int mlow = m0;
-        if (m0 > 1) { int a = first_rad_gt(P, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
+        if (m0 > 1) { int a = first_rad_gt(C, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
         const double ZZon = pk.ZZon;
         Strain s0 = kernel_strain<COOP>(P, C, m0, mlow, mupp);
         if (!finite6(s0.E)) { status = DUMP_NONFINITE; goto nocol; }
@@ -456,11 +490,11 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 double rupp_mp = P.rad(mp) + 2.0;
                 double rlow_mp = fmax(P.rad(mp) - 2.0, 0.0);
                 if (P.rad(mupp) > rupp_mp) {
-                    int b = last_rad_lt(P, mp + 1, mupp, rupp_mp);
+                    int b = last_rad_lt(C, mp + 1, mupp, rupp_mp);
                     mupp = b >= mp + 1 ? b : mp;
                 }
                 if (mlow > 1 && P.rad(mlow - 1) > rlow_mp) {
-                    int a = first_rad_gt(P, 1, mlow - 1, rlow_mp);
+                    int a = first_rad_gt(C, 1, mlow - 1, rlow_mp);
                     mlow = a <= mlow - 1 ? a : mlow;
                 }
                 Strain smp = kernel_strain<COOP>(P, C, mp, mlow, mupp);
@@ -547,7 +581,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         }
         if (Fbarx * RTHL5 > 0.0) Srb /= (Fbarx * RTHL5);
         // cells with r <= RTHL: shells 1..q
-        const int q = last_rad_lt(P, 1, m, nextafter(RTHL, 1e300));
+        const int q = last_rad_lt(C, 1, m, nextafter(RTHL, 1e300));
 
         if constexpr (FINISH) {
             double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
@@ -612,15 +646,15 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 double rupp_rc = rcur + 2.0;
                 int mupp_rc = 1;
                 if (m - 1 >= 2) {
-                    int f = first_rad_gt(P, 2, m - 1, rupp_rc);
+                    int f = first_rad_gt(C, 2, m - 1, rupp_rc);
                     mupp_rc = f <= m - 1 ? f : m - 1;
                 }
                 double rlow_rc = fmax(rcur - 2.0, 0.0);
                 int mlow_rc = 1;
-                if (m - 1 >= 1) { int f = first_rad_gt(P, 1, m - 1, rlow_rc); if (f <= m - 1) mlow_rc = f; }
+                if (m - 1 >= 1) { int f = first_rad_gt(C, 1, m - 1, rlow_rc); if (f <= m - 1) mlow_rc = f; }
                 int m0_rc = 1;
                 if (m - 1 >= 1) {
-                    int f = first_rad_gt(P, 1, m - 1, rcur);
+                    int f = first_rad_gt(C, 1, m - 1, rcur);
                     m0_rc = f <= m - 1 ? f : m - 1;
                 }
                 Strain src = kernel_strain<COOP>(P, C, m0_rc, mlow_rc, mupp_rc);

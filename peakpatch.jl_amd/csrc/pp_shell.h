@@ -44,7 +44,5 @@ int run_plain(pp_ctx *ctx, const GatherFields &G, ShellRun &S, PostConst &C, std
 int run_persistent(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C);
 int persistent_profiles(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, std::vector<int32_t> &cnt,
                         std::vector<double> &fbar, std::vector<int64_t> &off);
-int autotune_persistent(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks, double alatt,
-                        double rmax2rs, int *choice);
 
 }  // namespace pp

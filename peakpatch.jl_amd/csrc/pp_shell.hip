@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "pp_common.h"
@@ -123,20 +124,15 @@ struct GlobalProfile {
     __device__ double S2(int s, int L) const { return (double)f[(PF_S2 + L) * m + s - 1]; }
     __device__ double lap(int s) const { return (double)f[PF_L * m + s - 1]; }
     __device__ bool has_lapd() const { return lapd; }
-    __device__ void strain_inputs(int s, double G[3], double Gf[3], double Es[6]) const
+    __device__ void strain_raw(int s, float G[3], float Gf[3], float Es[6]) const
     {
-        if (s == 1) {
-            for (int i = 0; i < 3; i++) G[i] = Gf[i] = 0.0;
-            for (int i = 0; i < 6; i++) Es[i] = 0.0;
-            return;
-        }
-        const double r = radt[s - 1];
+        // centre shell sums are zero (RadialShell.jl:357-364)
         for (int i = 0; i < 3; i++) {
-            G[i] = (double)f[(PF_G + i) * m + s - 1] / r;
-            Gf[i] = (double)f[(PF_GF + i) * m + s - 1] / r;
+            G[i] = f[(PF_G + i) * m + s - 1];
+            Gf[i] = f[(PF_GF + i) * m + s - 1];
         }
-        double SR[9];
-        for (int i = 0; i < 9; i++) SR[i] = (double)f[(PF_SR + i) * m + s - 1] / r;
+        float SR[9];
+        for (int i = 0; i < 9; i++) SR[i] = f[(PF_SR + i) * m + s - 1];
         Es[0] = SR[0] + SR[0]; Es[1] = SR[4] + SR[4]; Es[2] = SR[8] + SR[8];
         Es[3] = SR[1] + SR[3]; Es[4] = SR[2] + SR[6]; Es[5] = SR[5] + SR[7];
     }
@@ -260,6 +256,18 @@ __global__ void k_fill_i32(int *p, int n, int v)
     if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
+// kernel_strain weight rows (one block per row)
+__global__ void k_wtab(const double *__restrict__ rad, const double *__restrict__ inv_rad,
+                       const double *__restrict__ akk, const int4 *__restrict__ rows, int nrows,
+                       double *__restrict__ w)
+{
+    for (int mp = blockIdx.x + 1; mp <= nrows; mp += gridDim.x) {
+        const int4 r = rows[mp - 1];
+        for (int m1 = r.y + (int)threadIdx.x; m1 <= r.z; m1 += blockDim.x)
+            w[r.x + m1 - r.y] = strain_weight(rad, inv_rad, akk, mp, m1);
+    }
+}
+
 // ------------------------------------------------------------- host side
 static double akk_host[1101];
 static void build_akk()
@@ -280,18 +288,69 @@ static void build_akk()
 int prepare_post_const(pp_ctx *ctx, int n, int nbuff, double alatt, const float *lapd, int8_t *mask, PostConst &C)
 {
     ShellTable &T = ctx->shells;
-    if (T.d_rad.bytes < (size_t)T.nshells * sizeof(double) || T.rad_rmax != T.rmax) {
-        std::vector<double> rad(T.nshells);
-        for (int s = 0; s < T.nshells; s++) rad[s] = std::sqrt((double)T.shell_r2[s]);
+    if (T.d_rad.bytes < 2 * (size_t)T.nshells * sizeof(double) || T.rad_rmax != T.rmax) {
+        std::vector<double> rad(2 * (size_t)T.nshells);
+        for (int s = 0; s < T.nshells; s++) {
+            rad[s] = std::sqrt((double)T.shell_r2[s]);
+            rad[T.nshells + s] = 1.0 / rad[s];
+        }
         PP_CHECK(T.d_rad.grow(rad.size() * sizeof(double)));
         PP_HIP(hipMemcpy(T.d_rad.ptr, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice));
         build_akk();
         PP_CHECK(T.d_akk.grow(sizeof akk_host));
         PP_HIP(hipMemcpy(T.d_akk.ptr, akk_host, sizeof akk_host, hipMemcpyHostToDevice));
+        // weight rows: mp > 1 keeps the m1 with (rad(mp) - rad(m1))^2 < 4
+        // (a contiguous run, rad being increasing); the centre row (mp = 1)
+        // has no such cut in the reference, and its callers' windows end at
+        // the last shell with rad <= 2 (r2 <= 4), which the row covers.
+        const int ns = T.nshells;
+        std::vector<int4> rows(ns);
+        int64_t tot = 0;
+        int lo = 1, hi = 1;
+        for (int mp = 1; mp <= ns; mp++) {
+            int a, b;
+            if (mp == 1) {
+                a = 1;
+                b = 1;
+                while (b + 1 <= ns && T.shell_r2[b] <= 4) b++;
+            } else {
+                const double rmp = rad[mp - 1];
+                auto in = [&](int m1) { const double u0 = rmp - rad[m1 - 1]; return u0 * u0 < 4.0; };
+                while (lo < mp && !in(lo)) lo++;
+                if (hi < mp) hi = mp;
+                while (hi + 1 <= ns && in(hi + 1)) hi++;
+                a = lo;
+                b = hi;
+            }
+            if (tot + (b - a + 1) > INT32_MAX) { set_error("kernel weight table too large"); return PP_ERR_CAPACITY; }
+            rows[mp - 1] = make_int4((int)tot, a, b, 0);
+            tot += b - a + 1;
+        }
+        const int r2max = T.shell_r2[ns - 1];
+        std::vector<int32_t> ge((size_t)r2max + 2);
+        for (int k = 0, sh = 1; k <= r2max + 1; k++) {
+            while (sh <= ns && T.shell_r2[sh - 1] < k) sh++;
+            ge[k] = sh;
+        }
+        PP_CHECK(T.d_shell_ge.grow(ge.size() * sizeof(int32_t)));
+        PP_HIP(hipMemcpy(T.d_shell_ge.ptr, ge.data(), ge.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        PP_CHECK(T.d_wrow.grow(rows.size() * sizeof(int4)));
+        PP_CHECK(T.d_wtab.grow((size_t)tot * sizeof(double)));
+        PP_HIP(hipMemcpy(T.d_wrow.ptr, rows.data(), rows.size() * sizeof(int4), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_wtab, dim3(std::min(ns, 4096)), dim3(256), 0, ctx->stream, T.d_rad.as<double>(),
+                           T.d_rad.as<double>() + ns, T.d_akk.as<double>(), T.d_wrow.as<int4>(), ns,
+                           T.d_wtab.as<double>());
+        PP_HIP(hipGetLastError());
         T.rad_rmax = T.rmax;
     }
     C.rad = T.d_rad.as<double>();
+    C.inv_rad = T.d_rad.as<double>() + T.nshells;
     C.akk = T.d_akk.as<double>();
+    C.wrow = T.d_wrow.as<int4>();
+    C.shell_ge = T.d_shell_ge.as<int32_t>();
+    C.r2max = T.shell_r2[T.nshells - 1];
+    C.nshells = T.nshells;
+    C.wtab = T.d_wtab.as<double>();
     C.cell = T.d_cell.as<int32_t>();
     C.shell_start = T.d_shell_start.as<int32_t>();
     C.tab.data = ctx->d_table.as<float>();
@@ -489,6 +548,7 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     PP_CHECK(check_fields(ctx, fields));
     PP_CHECK(ensure_device(ctx));
     for (int q = 0; q < 7; q++) out->dump_counts[q] = 0;
+    for (double &t : ctx->timing.stage_ms) t = 0.0;
     const int64_t np = peaks->npeaks;
     if (np == 0) return PP_OK;
     PP_ARG(peaks->i && peaks->j && peaks->k && peaks->Rf && peaks->ir2min && peaks->ZZon && peaks->fcrit,
@@ -584,10 +644,44 @@ int pp_shell_profiles(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
     return PP_OK;
 }
 
+// Coordinate descent over the persistent form's launch parameters, each
+// candidate timed (HIP events around the whole shell call, best of two runs)
+// on the caller's batch; the fastest setting stays in the context.
 int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *peaks, double alatt,
                       double rmax2rs, int *choice)
 {
-    return autotune_persistent(ctx, fields, peaks, alatt, rmax2rs, choice);
+    PP_ARG(ctx && fields && peaks, "NULL argument");
+    pp_results res;
+    memset(&res, 0, sizeof res);
+    auto timed = [&](double &best) -> int {
+        best = 1e300;
+        for (int rep = 0; rep < 2; rep++) {
+            PP_CHECK(pp_analyse_peaks(ctx, fields, peaks, alatt, rmax2rs, 0, nullptr, 0, PP_SHELL_PERSISTENT, &res));
+            best = std::min(best, ctx->timing.shell_ms);
+        }
+        return PP_OK;
+    };
+    double t = 0;
+    PP_CHECK(timed(t));  // warm-up: tables, buffers
+    static const int cand_small[] = {0, 8, 12, 16};
+    static const int cand_chunk[] = {32, 64, 128, 256};
+    static const int cand_coop[] = {32, 64, 128, 256, 1 << 20};
+    static const int cand_waves[] = {8, 12, 16, 24, 32};
+    struct Knob { int *v; const int *c; int nc; };
+    const Knob knobs[] = {{&ctx->chunk_shells, cand_chunk, 4}, {&ctx->coop_m, cand_coop, 5},
+                          {&ctx->gather_waves_cu, cand_waves, 5}, {&ctx->tiny_m, cand_small, 4}};
+    for (const Knob &k : knobs) {
+        int bestv = *k.v;
+        double bt = 1e300;
+        for (int q = 0; q < k.nc; q++) {
+            *k.v = k.c[q];
+            PP_CHECK(timed(t));
+            if (t < bt) { bt = t; bestv = k.c[q]; }
+        }
+        *k.v = bestv;
+    }
+    if (choice) *choice = ctx->gather_waves_cu;
+    return PP_OK;
 }
 
 }  // extern "C"

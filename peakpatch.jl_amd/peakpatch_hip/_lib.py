@@ -76,6 +76,8 @@ SIGNATURES = {
     "pp_shell_profiles": (_i, [_vp, C.POINTER(Fields), C.POINTER(PeakBatch), _d, _d, _i, _i, _vp, _vp, _vp]),
     "pp_autotune_shell": (_i, [_vp, C.POINTER(Fields), C.POINTER(PeakBatch), _d, _d, C.POINTER(_i)]),
     "pp_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
+    "pp_shell_tunables": (_i, [_vp, _vp, _vp]),
+    "pp_last_shell_stages": (_i, [_vp, C.POINTER(_d)]),
 }
 
 _lib = None

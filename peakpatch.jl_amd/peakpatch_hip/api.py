@@ -108,10 +108,34 @@ class Context:
     def synchronize(self):
         L.check(self.lib.pp_ctx_synchronize(self.h))
 
+    TUNABLES = ("small_m", "chunk_shells", "coop_m", "gather_waves_cu")
+
+    def tunables(self, **kw):
+        """Get (and with keywords, set) the persistent form's launch
+        parameters: small_m, chunk_shells, coop_m, gather_waves_cu."""
+        unknown = set(kw) - set(self.TUNABLES)
+        if unknown:
+            raise ValueError(f"unknown tunables {sorted(unknown)}")
+        get = np.zeros(4, np.int32)
+        if kw:
+            st = np.array([-1, 0, 0, 0], np.int32)
+            for q, name in enumerate(self.TUNABLES):
+                if name in kw:
+                    st[q] = int(kw[name])
+            L.check(self.lib.pp_shell_tunables(self.h, st.ctypes.data, get.ctypes.data))
+        else:
+            L.check(self.lib.pp_shell_tunables(self.h, None, get.ctypes.data))
+        return dict(zip(self.TUNABLES, (int(v) for v in get)))
+
+    STAGES = ("prepare", "gather", "small_gather", "post_wave", "post_lane", "finish")
+
     def last_timing(self):
         g = C.c_double(); s = C.c_double(); p = C.c_double()
         L.check(self.lib.pp_last_timing(self.h, C.byref(g), C.byref(s), C.byref(p)))
-        return {"gather_ms": g.value, "shell_ms": s.value, "peakfind_ms": p.value}
+        st = (C.c_double * 6)()
+        L.check(self.lib.pp_last_shell_stages(self.h, st))
+        return {"gather_ms": g.value, "shell_ms": s.value, "peakfind_ms": p.value,
+                "stages_ms": dict(zip(self.STAGES, list(st)))}
 
     # ------------------------------------------------------- filter bank
     def window_multiply(self, delta_k, n: int, boxsize: float, Rf: float, wsmooth: int, scale: float = 1.0):
@@ -265,6 +289,16 @@ class Context:
             out["strain"] = out["strain"].reshape(n, 3, 3, order="C").transpose(0, 2, 1)
         out["dump_counts"] = dict(zip(DUMP_NAMES, list(r.dump_counts)))
         return out
+
+    def autotune_shell(self, delta, eta, i, j, k, Rf, ir2min, ZZon, fcrit, alatt: float, eta2=None, lapd=None,
+                       rmax2rs: float = 0.0) -> dict:
+        """Time the persistent form on this batch over its launch parameters
+        and keep the fastest in the context; returns the chosen tunables."""
+        f, keep = self._fields(delta, eta, eta2, lapd)
+        b, keepb = self._batch(i, j, k, Rf, ir2min, ZZon, fcrit)
+        choice = C.c_int()
+        L.check(self.lib.pp_autotune_shell(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, C.byref(choice)))
+        return self.tunables()
 
     def shell_profiles(self, delta, eta, i, j, k, Rf, alatt: float, eta2=None, lapd=None,
                        rmax2rs: float = 0.0, max_shells: int = 64, kernel: int = L.SHELL_PLAIN):

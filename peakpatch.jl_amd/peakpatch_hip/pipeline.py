@@ -177,6 +177,9 @@ class TileDriver:
         of the algorithmic byte count."""
         return gather_cells(self.cfg.nhunt, b.Rf, self.cfg.alatt, self.cfg.rmax2rs)
 
+    def gather_extent(self, b: PeakBatch) -> tuple[np.ndarray, np.ndarray]:
+        return gather_extent(self.cfg.nhunt, b.Rf, self.cfg.alatt, self.cfg.rmax2rs)
+
 
 def shell_counts(rmax: int) -> tuple[np.ndarray, np.ndarray]:
     """(r2 of each shell, cells per shell) of precompute_shells(rmax)."""
@@ -188,6 +191,12 @@ def shell_counts(rmax: int) -> tuple[np.ndarray, np.ndarray]:
 
 
 def gather_cells(rmax: int, Rf, alatt: float, rmax2rs: float) -> np.ndarray:
+    return gather_extent(rmax, Rf, alatt, rmax2rs)[0]
+
+
+def gather_extent(rmax: int, Rf, alatt: float, rmax2rs: float) -> tuple[np.ndarray, np.ndarray]:
+    """(cells read, finalised shells m) per peak: the centre plus shells
+    2..m, m = shell(npart - 1) - 1 (RadialShell.jl:289-295, 379-453)."""
     r2, cnt = shell_counts(rmax)
     start = np.concatenate([[0], np.cumsum(cnt)])          # start[s-1] = first cell of shell s
     ncells = start[-1]
@@ -200,4 +209,4 @@ def gather_cells(rmax: int, Rf, alatt: float, rmax2rs: float) -> np.ndarray:
     # m = shell(npart-1) - 1 finalised shells; cells read = start[m] (shells 1..m)
     shell_of_last = np.searchsorted(start, np.maximum(npart - 1, 0), side="right")
     m = np.where(npart >= 2, shell_of_last - 1, 1)
-    return start[m]
+    return start[m], m

@@ -113,10 +113,9 @@ def test_shell_profile_counts_bitexact(ctx, form):
         assert np.allclose(fb[q, :m[q]], prof["Fbar"][:m[q]], rtol=1e-6, atol=1e-6)
 
 
-def test_forms_agree_on_unsorted_mixed_batch(ctx):
-    """Plain and persistent forms on a shuffled batch mixing tiny, gathered
-    and boundary peaks (exercises the radix sort, the class launches and the
-    tiny-peak lanes), against each other and the oracle."""
+def mixed_case(ctx):
+    """A shuffled batch mixing small, wave-gathered and boundary peaks on a
+    48^3 bump field, with its oracle results and mask."""
     n, rmax = 48, 12
     d, eta, eta2, lapd = bump_field(n, 99, nbump=25, noise=0.3)
     ct = collapse.sinusoidal_table()
@@ -133,20 +132,70 @@ def test_forms_agree_on_unsorted_mixed_batch(ctx):
     fc = 0.6 + 0.3 * rng.random(npk)
     perm = rng.permutation(npk)
     i, j, k, Rf, ir2, ZZ, fc = (a[perm] for a in (i, j, k, Rf, ir2, ZZ, fc))
-    out = {}
-    masks = {}
-    for form in FORMS:
-        masks[form] = np.zeros(d.shape, np.int8)
-        out[form] = ctx.analyse_peaks(d, eta, i, j, k, Rf, ir2, ZZ, fc, 1.0, eta2=eta2, lapd=lapd, rmax2rs=2.0,
-                                      nbuff=3, mask=masks[form], kernel=form)
     mask_c = np.zeros(d.shape, np.int8)
     ref, st = O.analyse_peaks(O.Grid(d, *eta, eta2=eta2, mask=mask_c, lapd=lapd), ipp1(i, j, k, n), ir2, ZZ, Rf, fc,
                               1.0, tab, O.Shells(rmax), 3, 2.0)
+
+    def run(form):
+        mask = np.zeros(d.shape, np.int8)
+        r = ctx.analyse_peaks(d, eta, i, j, k, Rf, ir2, ZZ, fc, 1.0, eta2=eta2, lapd=lapd, rmax2rs=2.0, nbuff=3,
+                              mask=mask, kernel=form)
+        return r, mask
+    run.autotune = lambda: ctx.autotune_shell(d, eta, i, j, k, Rf, ir2, ZZ, fc, 1.0, eta2=eta2, lapd=lapd,
+                                              rmax2rs=2.0)
+    return run, ref, st, mask_c
+
+
+def test_forms_agree_on_unsorted_mixed_batch(ctx):
+    """Plain and persistent forms on a shuffled batch mixing small, gathered
+    and boundary peaks (exercises the radix sort, the small-peak waves, the
+    chunked gather and both posts), against each other and the oracle."""
+    run, ref, st, mask_c = mixed_case(ctx)
     assert (st == 0).sum() > 100 and (st == 6).sum() > 0
+    out = {}
     for form in FORMS:
+        out[form], mask = run(form)
         compare(out[form], ref, st)
-        assert np.array_equal(masks[form], mask_c)
+        assert np.array_equal(mask, mask_c)
         assert np.array_equal(out[form]["nshells"], out[FORMS[0]]["nshells"])
+
+
+TUNINGS = [
+    dict(small_m=0, chunk_shells=8, coop_m=4, gather_waves_cu=4),       # everything through the chunked gather
+    dict(small_m=16, chunk_shells=16, coop_m=1 << 20, gather_waves_cu=32),  # lane posts only
+    dict(small_m=6, chunk_shells=1024, coop_m=1, gather_waves_cu=1),    # one chunk, wave posts only
+]
+
+
+@pytest.mark.parametrize("tune", TUNINGS, ids=["gather-all", "lane-post", "wave-post"])
+def test_persistent_tunables_keep_results(ctx, tune):
+    """The persistent form's launch parameters move peaks between code paths
+    (small waves / chunk tables / lane or wave post); every setting must
+    still match the oracle."""
+    run, ref, st, mask_c = mixed_case(ctx)
+    before = ctx.tunables()
+    try:
+        assert ctx.tunables(**tune) == tune
+        r, mask = run(SHELL_PERSISTENT)
+    finally:
+        ctx.tunables(**before)
+    assert ctx.tunables() == before
+    compare(r, ref, st)
+    assert np.array_equal(mask, mask_c)
+
+
+def test_autotune_keeps_results(ctx):
+    """The autotuner picks one of its candidates and results stay put."""
+    run, ref, st, mask_c = mixed_case(ctx)
+    before = ctx.tunables()
+    try:
+        chosen = run.autotune()
+        assert chosen["chunk_shells"] in (32, 64, 128, 256) and chosen["small_m"] in (0, 8, 12, 16)
+        r, mask = run(SHELL_PERSISTENT)
+    finally:
+        ctx.tunables(**before)
+    compare(r, ref, st)
+    assert np.array_equal(mask, mask_c)
 
 
 @pytest.mark.parametrize("form", FORMS)
