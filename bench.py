@@ -106,6 +106,19 @@ def cpu_baseline(cfg, tile, b, ct, seconds: float):
                       f"{t:.1f} s, FP64 oracle restating RadialShell.analyse_peak"}
 
 
+def read_traffic(path, head, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    same counter on a copy kernel of known size), or None."""
+    if head != "persistent" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
+                return float(line.split("=", 1)[1].split()[0])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +130,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--tune", default="", help="pin launch parameters, e.g. small_m=12,chunk_shells=128 "
+                    "(implies --no-autotune)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="before the timed steps, copy a GIB-sized float32 buffer once (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_gather_c3_pmc.txt"),
+                    help="rocprofv3 --pmc summary whose 'traffic_bytes_per_launch' fills roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -150,6 +170,9 @@ def main():
     prop = torch.cuda.get_device_properties(0)
     log(f"[bench] device {prop.name}: {prop.multi_processor_count} CUs, {prop.total_memory / 2**30:.0f} GiB")
 
+    if args.tune:
+        ctx.tunables(**{k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))})
+        args.no_autotune = True
     tuned = ctx.tunables()
     if not args.no_autotune and args.form in ("auto", "persistent"):
         t0 = time.perf_counter()
@@ -158,6 +181,16 @@ def main():
                                    bdev["ZZon"], bdev["fcrit"], c.alatt, eta2=fields[2] if c.ilpt >= 2 else None,
                                    lapd=fields[3] if c.ioutshear >= 1 else None, rmax2rs=c.rmax2rs)
         log(f"[bench] autotuned launch parameters in {time.perf_counter() - t0:.1f} s: {tuned}")
+
+    if args.copy_calibration > 0:
+        nel = int(args.copy_calibration * 2**30) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {nel * 4} bytes read, {nel * 4} bytes written")
+        del src, dst
 
     forms = []
     if args.form in ("auto", "persistent"):
@@ -227,6 +260,7 @@ def main():
         out["plain_kernel_ms"] = float(np.mean(pk)) * 1e3
         out["plain_peaks_per_s"] = npk / float(np.mean(pw))
         out["speedup_vs_plain_kernel"] = float(np.mean(pk)) / k_s
+    out["roofline"]["traffic"] = read_traffic(args.traffic_file, head, dom_name)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, tile, b, ct, args.cpu_seconds)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")
