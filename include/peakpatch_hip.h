@@ -210,6 +210,34 @@ int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get);
 int pp_last_timing(pp_ctx *ctx, double *shell_gather_ms, double *shell_total_ms,
                    double *peakfind_ms);
 
+/* ------------------------------------------------------- halo records */
+/* Per-peak inputs of the record packing besides the pp_results: the host
+ * bookkeeping of the drivers (Pipeline.jl:217-231, 257-271).  Device arrays. */
+typedef struct {
+    int64_t npeaks;
+    const double *x, *y, *z;  /* peak position, Mpc/h (PeakCandidate x, y, z)  */
+    const double *Rf;         /* filter radius                                */
+    const double *ZZon;       /* 1 + z of the peak                            */
+    const double *D;          /* D(a_pk), Dlinear_ab                          */
+    const double *v2fac;      /* 3/7 Om_a^(-1/143) D^2 (Pipeline.jl:270-271);
+                                 NULL without 2LPT fields (zero velocities)   */
+    const float *FcRf;        /* smoothed delta at the peak                   */
+    const float *d2Rf;        /* wsmooth=3 field at the peak, may be NULL     */
+    int on_device;            /* must be 1                                    */
+} pp_halo_inputs;
+
+/* Record packing of run_tile / run_multitile_split (Pipeline.jl:247-308;
+ * MultiResolution.jl:1025-1094): every peak with RTHL > 0 and not
+ * z > z_max (pass +INFINITY for no cut), in peak order, becomes a HaloRecord (11 Float32) or, with
+ * extended, an ExtHaloRecord (33 Float32) in the .pksc field order
+ * (src/IO/Catalog.jl:48-94).  res must hold device results of
+ * pp_analyse_peaks for the same peaks.  *nhalo is the number of records
+ * (> capacity => PP_ERR_CAPACITY, nothing written); *RTHLmax the largest
+ * RTHL * alatt (0 when empty), the .pksc header value. */
+int pp_pack_halos(pp_ctx *ctx, const pp_results *res, const pp_halo_inputs *in, double alatt, double z_max,
+                  int extended, float *records, int64_t capacity, int records_on_device, int64_t *nhalo,
+                  float *RTHLmax);
+
 /* Per-stage device time of the last persistent-form call, milliseconds
  * (HIP events on the context stream): ms[0] preparation (transpose, order,
  * offsets), ms[1] wave gather, ms[2] small-peak gather, ms[3] wave post,

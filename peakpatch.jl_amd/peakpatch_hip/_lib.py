@@ -53,6 +53,12 @@ class Results(C.Structure):
                 ("dump_counts", C.c_int64 * 7)]
 
 
+class HaloInputs(C.Structure):
+    _fields_ = [("npeaks", C.c_int64), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+                ("Rf", C.c_void_p), ("ZZon", C.c_void_p), ("D", C.c_void_p), ("v2fac", C.c_void_p),
+                ("FcRf", C.c_void_p), ("d2Rf", C.c_void_p), ("on_device", C.c_int)]
+
+
 _vp, _i, _d, _f, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int64
 
 SIGNATURES = {
@@ -78,6 +84,8 @@ SIGNATURES = {
     "pp_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
     "pp_shell_tunables": (_i, [_vp, _vp, _vp]),
     "pp_last_shell_stages": (_i, [_vp, C.POINTER(_d)]),
+    "pp_pack_halos": (_i, [_vp, C.POINTER(Results), C.POINTER(HaloInputs), _d, _d, _i, _vp, _i64, _i,
+                           C.POINTER(_i64), C.POINTER(_f)]),
 }
 
 _lib = None

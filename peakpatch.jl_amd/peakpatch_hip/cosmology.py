@@ -83,6 +83,12 @@ class GrowthTables:
     def D(self, a):
         """Dlinear(a) = a * D_a (first output of Dlinear_ab), vectorised."""
         a = np.atleast_1d(np.asarray(a, np.float64))
+        return self.D_a(a) * a
+
+    def D_a(self, a):
+        """D_a, the interpolated table value (third output of Dlinear_ab,
+        Cosmology.jl:273-293), vectorised; the drivers' velocity factor."""
+        a = np.atleast_1d(np.asarray(a, np.float64))
         at, Dt = self.atab, self.Datab
         n = len(at)
         kk = np.searchsorted(at, a, side="left")          # _hunt: at[kk-1] < a <= at[kk] (1-based kk)
@@ -110,10 +116,9 @@ class GrowthTables:
         x0, x1 = at[i0], at[np.minimum(i0 + 1, n - 1)]
         a1 = (x - x1) / (x0 - x1)
         D2 = a1 * Dt[i0] + (1.0 - a1) * Dt[np.minimum(i0 + 1, n - 1)]
-        Da = np.where(two, D2, D4)
-        out[inside] = Da * x
-        out[a >= at[-1]] = Dt[-1] * a[a >= at[-1]]
-        out[a <= at[0]] = Dt[0] * a[a <= at[0]]
+        out[inside] = np.where(two, D2, D4)
+        out[a >= at[-1]] = Dt[-1]
+        out[a <= at[0]] = Dt[0]
         return out
 
     def fsc_of_z(self, z):

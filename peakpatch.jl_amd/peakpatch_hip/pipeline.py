@@ -103,6 +103,8 @@ class PeakBatch:
     z: np.ndarray
     FcRf: np.ndarray
     d2Rf: np.ndarray
+    D: np.ndarray        # D_a at the peak (Dlinear_ab third output), 1LPT velocity factor
+    v2fac: np.ndarray    # 3/7 Om_a^(-1/143) D^2, 2LPT velocity factor (zeros without 2LPT)
 
     def __len__(self):
         return len(self.i)
@@ -116,7 +118,10 @@ class PeakBatch:
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
         return {"i": t(self.i, torch.int32), "j": t(self.j, torch.int32), "k": t(self.k, torch.int32),
                 "Rf": t(self.Rf, torch.float64), "ir2min": t(self.ir2min, torch.int64),
-                "ZZon": t(self.ZZon, torch.float64), "fcrit": t(self.fcrit, torch.float64)}
+                "ZZon": t(self.ZZon, torch.float64), "fcrit": t(self.fcrit, torch.float64),
+                "x": t(self.x, torch.float64), "y": t(self.y, torch.float64), "z": t(self.z, torch.float64),
+                "FcRf": t(self.FcRf, torch.float32), "d2Rf": t(self.d2Rf, torch.float32),
+                "D": t(self.D, torch.float64), "v2fac": t(self.v2fac, torch.float64)}
 
 
 class TileDriver:
@@ -155,9 +160,19 @@ class TileDriver:
             zpk = self.chi2z.peak_redshift(c.cen, x, y, z)
         else:
             zpk = np.full(len(x), c.z_out)
+        a_pk = 1.0 / (1.0 + zpk)
+        D_pk = self.growth.D_a(a_pk)
+        if c.ilpt >= 2:
+            # Pipeline.jl:268-271: -Sbar2 .* (-3/7 Om_a^(-1/143) D^2); the
+            # two negations are exact, so the factor is kept positive
+            Om, OL = c.cosmology.Om, c.cosmology.OL
+            Om_a = Om * a_pk ** 3 / (Om * a_pk ** 3 + OL)
+            v2 = -(-3.0 / 7.0 * Om_a ** (-1.0 / 143) * D_pk ** 2)
+        else:
+            v2 = np.zeros(len(x))
         b = PeakBatch(peaks.i, peaks.j, peaks.k, Rf, cosmo_mod.ir2min(Rf, c.alatt), 1.0 + zpk,
                       self.growth.fsc_of_z(zpk), x, y, z, peaks.delta_s,
-                      peaks.lapd_s if peaks.lapd_s is not None else np.zeros(len(x), np.float32))
+                      peaks.lapd_s if peaks.lapd_s is not None else np.zeros(len(x), np.float32), D_pk, v2)
         if c.ievol == 1:
             b = b.take(zpk <= c.z_max)
         return b
