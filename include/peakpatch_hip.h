@@ -68,6 +68,10 @@ int pp_device_count(int *count);
 int pp_ctx_create(int device, pp_ctx **out);
 int pp_ctx_destroy(pp_ctx *ctx);
 int pp_ctx_synchronize(pp_ctx *ctx);
+/* The context's HIP stream (a hipStream_t), so a caller can order its own
+ * device work (input synthesis, copies) on the same queue.  No counterpart in
+ * the reference (CUDA.jl's task-local stream). */
+int pp_ctx_stream(pp_ctx *ctx, void **stream);
 /* Free / total device memory as seen by this context (for admission). */
 int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 

@@ -159,6 +159,13 @@ int pp_ctx_synchronize(pp_ctx *ctx)
     return PP_OK;
 }
 
+int pp_ctx_stream(pp_ctx *ctx, void **stream)
+{
+    PP_ARG(ctx && stream, "NULL argument");
+    *stream = (void *)ctx->stream;
+    return PP_OK;
+}
+
 int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes)
 {
     PP_ARG(ctx && free_bytes && total_bytes, "NULL argument");

@@ -105,6 +105,13 @@ class Context:
         L.check(self.lib.pp_ctx_mem_info(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
 
+    def stream_handle(self) -> int:
+        """The context's HIP stream as an integer handle
+        (torch.cuda.ExternalStream accepts it)."""
+        s = C.c_void_p()
+        L.check(self.lib.pp_ctx_stream(self.h, C.byref(s)))
+        return int(s.value or 0)
+
     def synchronize(self):
         L.check(self.lib.pp_ctx_synchronize(self.h))
 

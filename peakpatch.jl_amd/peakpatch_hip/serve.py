@@ -1,0 +1,304 @@
+"""Tile-serving loop: a stream of independent light-cone tiles of uneven
+nmesh, the unit of work of run_multitile_split (src/MultiResolution.jl:
+719-1180, one `process_tile!` per tile; round-robin dispatch :1183-1203).
+
+Design
+------
+* Slots.  Each slot owns a Context (its own HIP stream, pooled device
+  buffers, the collapse table and the shell tables, all resident for the
+  server's life) and one worker thread.  Everything a tile does -- input
+  synthesis (torch on the slot's stream), filter bank + stencil, shell
+  analysis, record packing -- is queued on that one stream, so tiles on
+  different slots overlap on the device while each tile stays in order.
+* Admission by free HBM.  A tile's device footprint is estimated from its
+  nmesh (``footprint_bytes``).  The scheduler admits the oldest waiting tile
+  when a slot is idle and the footprints of the tiles in flight plus this one
+  fit the budget (a fraction of the HBM free when the server started).  A
+  tile that could never fit is failed at once instead of blocking the queue.
+* Continuous batching.  Requests arriving while others run are admitted as
+  soon as a slot and memory free up; the device interleaves their kernels.
+  Results do not depend on what else runs: every kernel of the path is
+  deterministic and a tile touches only its own slot's buffers.
+* Latency = completion - arrival (queueing included); the stream summary
+  reports tiles/s and p50/p99.
+
+The tile pipeline is injectable (``run_tile``) so the scheduler's policies
+and failure handling are testable without a GPU.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import math
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .pipeline import TileConfig
+
+# c5 tiles: the c2 physics at fixed resolution (alatt = 250/256 Mpc/h), so
+# every tile shares one shell table whatever its nmesh; light-cone redshifts
+# from an observer at the origin.
+SERVE_TEMPLATE = TileConfig("c5", 256, 250.0, 20, 1.0, 15.0, ilpt=2, ioutshear=1, ievol=1, z_max=4.6,
+                            cen=(0.0, 0.0, 0.0))
+SERVE_NMESH = (128, 256, 384, 512, 768)
+
+
+@dataclass
+class TileRequest:
+    tile_id: int
+    nmesh: int
+    center: tuple = (0.0, 0.0, 0.0)
+    seed: int = 42
+    arrival: float = 0.0            # perf_counter() at submission (set by submit)
+
+
+@dataclass
+class TileResult:
+    tile_id: int
+    nmesh: int
+    records: np.ndarray | None = None   # (nhalo, 33) float32 ExtHaloRecord
+    RTHLmax: float = 0.0
+    npeaks: int = 0
+    nhalo: int = 0
+    arrival: float = 0.0
+    start: float = 0.0
+    done: float = 0.0
+    slot: int = -1
+    error: str | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+def tile_config(req: TileRequest, template: TileConfig = SERVE_TEMPLATE) -> TileConfig:
+    alatt = template.alatt
+    return replace(template, name=f"{template.name}/tile{req.tile_id}", n=req.nmesh, boxsize=req.nmesh * alatt,
+                   tile_center=tuple(req.center), seed=req.seed)
+
+
+def footprint_bytes(n: int) -> int:
+    """Device bytes one tile holds at its peak: the 9 Float32 fields, their
+    spectrum and the synthesis temporaries (~24 B/cell), filter scratch and
+    mask (~9 B/cell), the transposed delta, profiles and results
+    (~30 B/cell at ~1.5e-2 peaks/cell), plus fixed tables and plans."""
+    return int(n) ** 3 * 100 + (256 << 20)
+
+
+def poisson_stream(ntiles: int, rate: float, seed: int = 0, nmesh=SERVE_NMESH, weights=None,
+                   extent: float = 1500.0):
+    """(arrival offset s, TileRequest) pairs: exponential gaps at `rate`
+    tiles/s, nmesh drawn from `nmesh`, tile centres uniform in a light-cone
+    cube of side `extent` Mpc/h beside the observer."""
+    rng = np.random.default_rng(seed)
+    w = None if weights is None else np.asarray(weights, float) / np.sum(weights)
+    t = 0.0
+    out = []
+    for q in range(ntiles):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = int(rng.choice(nmesh, p=w))
+        c = tuple(float(v) for v in rng.uniform(0.0, extent, 3))
+        out.append((t, TileRequest(q, n, c, seed=1000 + q)))
+    return out
+
+
+class AdmissionError(RuntimeError):
+    pass
+
+
+@dataclass
+class _Slot:
+    index: int
+    state: object = None           # what run_tile needs (Context, stream, ...)
+    busy: bool = False
+
+
+class TileServer:
+    """Continuous-batching tile server.  ``run_tile(slot_state, request)``
+    returns a TileResult; by default the GPU pipeline of ``GpuTileRunner``."""
+
+    def __init__(self, slots: int = 4, budget_bytes: int | None = None, run_tile=None, slot_states=None,
+                 footprint=footprint_bytes, device: int = 0, hbm_fraction: float = 0.8):
+        self._footprint = footprint
+        if run_tile is None:
+            runner = GpuTileRunner(device)
+            run_tile = runner.run
+            slot_states = [runner.make_slot() for _ in range(slots)]
+            if budget_bytes is None:
+                free, _ = slot_states[0]["ctx"].mem_info()
+                budget_bytes = int(free * hbm_fraction)
+            self._runner = runner
+        else:
+            self._runner = None
+            if slot_states is None:
+                slot_states = [None] * slots
+        self._run = run_tile
+        self.budget = budget_bytes if budget_bytes is not None else 1 << 62
+        self._slots = [_Slot(q, s) for q, s in enumerate(slot_states)]
+        self._pending = collections.deque()
+        self._reserved = 0
+        self._cv = threading.Condition()
+        self._closed = False
+        self.max_in_flight = 0
+        self._in_flight = 0
+        self._sched = threading.Thread(target=self._schedule, name="tile-scheduler", daemon=True)
+        self._sched.start()
+
+    # ---------------------------------------------------------------- API
+    def submit(self, req: TileRequest) -> Future:
+        fut: Future = Future()
+        req.arrival = time.perf_counter()
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("server closed")
+            self._pending.append((req, fut))
+            self._cv.notify_all()
+        return fut
+
+    def close(self):
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        self._sched.join()
+        # wait for running tiles
+        with self._cv:
+            while self._in_flight:
+                self._cv.wait()
+        if self._runner is not None:
+            for s in self._slots:
+                self._runner.release_slot(s.state)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------- scheduler
+    def _schedule(self):
+        with self._cv:
+            while True:
+                while self._pending:
+                    req, fut = self._pending[0]
+                    need = self._footprint(req.nmesh)
+                    if need > self.budget:
+                        self._pending.popleft()
+                        fut.set_result(TileResult(req.tile_id, req.nmesh, arrival=req.arrival,
+                                                  start=time.perf_counter(), done=time.perf_counter(),
+                                                  error=f"AdmissionError: tile needs {need} B of HBM, "
+                                                        f"budget {self.budget} B"))
+                        continue
+                    slot = next((s for s in self._slots if not s.busy), None)
+                    if slot is None or self._reserved + need > self.budget:
+                        break  # FIFO: the head waits for a slot / memory
+                    self._pending.popleft()
+                    slot.busy = True
+                    self._reserved += need
+                    self._in_flight += 1
+                    self.max_in_flight = max(self.max_in_flight, self._in_flight)
+                    threading.Thread(target=self._work, args=(slot, req, fut, need), daemon=True,
+                                     name=f"tile-slot{slot.index}").start()
+                if self._closed and not self._pending:
+                    return
+                self._cv.wait()
+
+    def _work(self, slot: _Slot, req: TileRequest, fut: Future, need: int):
+        start = time.perf_counter()
+        try:
+            res = self._run(slot.state, req)
+        except Exception as e:  # the tile fails, the server keeps serving
+            res = TileResult(req.tile_id, req.nmesh, error=f"{type(e).__name__}: {e}")
+        res.arrival, res.start, res.slot = req.arrival, start, slot.index
+        res.done = time.perf_counter()
+        with self._cv:
+            slot.busy = False
+            self._reserved -= need
+            self._in_flight -= 1
+            self._cv.notify_all()
+        fut.set_result(res)
+
+
+def replay(server: TileServer, stream, time_scale: float = 1.0):
+    """Submit (offset, request) pairs at their offsets; return the results in
+    request order once all have finished."""
+    t0 = time.perf_counter()
+    futs = []
+    for off, req in stream:
+        dt = t0 + off * time_scale - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(server.submit(req))
+    return [f.result() for f in futs]
+
+
+def summarize(results) -> dict:
+    ok = [r for r in results if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+    wall = (max(r.done for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
+    return {"tiles": len(results), "completed": len(ok), "failed": len(results) - len(ok),
+            "tiles_per_s": len(ok) / wall if wall > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat, 50)) * 1e3 if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) * 1e3 if len(lat) else None,
+            "peaks": int(sum(r.npeaks for r in ok)), "halos": int(sum(r.nhalo for r in ok)),
+            "wall_s": wall}
+
+
+class GpuTileRunner:
+    """The per-tile GPU pipeline on one slot: synthesis -> filter bank +
+    stencil -> per-peak bookkeeping -> shell analysis -> record packing, all
+    on the slot context's stream."""
+
+    def __init__(self, device: int = 0, template: TileConfig = SERVE_TEMPLATE, table=None, tunables=None):
+        from . import collapse
+        self.device = device
+        self.template = template
+        self.table = table if table is not None else collapse.sinusoidal_table()
+        self.tunables = tunables
+
+    def make_slot(self):
+        import torch
+        from .api import Context
+        ctx = Context(self.device)
+        ctx.upload_collapse_table(self.table.table, *self.table.bounds)
+        ctx.build_shell_tables(self.template.nhunt)
+        if self.tunables:
+            ctx.tunables(**self.tunables)
+        stream = torch.cuda.ExternalStream(ctx.stream_handle(), device=torch.device("cuda", self.device))
+        return {"ctx": ctx, "stream": stream}
+
+    @staticmethod
+    def release_slot(state):
+        if state is not None:
+            state["ctx"].close()
+
+    def run(self, state, req: TileRequest) -> TileResult:
+        import torch
+        from . import catalog, synth
+        from ._lib import SHELL_PERSISTENT
+        from .pipeline import TileDriver
+        ctx, stream = state["ctx"], state["stream"]
+        cfg = tile_config(req, self.template)
+        drv = TileDriver(cfg)
+        with torch.cuda.stream(stream):
+            tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt,
+                                         want_lapd=cfg.ioutshear >= 1, device=f"cuda:{self.device}")
+            peaks = drv.find_peaks(ctx, tile.delta)
+            b = drv.batch(peaks)
+            out = TileResult(req.tile_id, req.nmesh, npeaks=len(b))
+            if len(b) == 0:
+                out.records = np.zeros((0, 33), np.float32)
+                return out
+            bdev = b.to_device(f"cuda:{self.device}")
+            mask = torch.zeros((cfg.n,) * 3, dtype=torch.int8, device=f"cuda:{self.device}")
+            fields = (tile.delta, tile.eta, tile.eta2, tile.lapd)
+            res = drv.analyse(ctx, fields, bdev, mask=mask, kernel=SHELL_PERSISTENT, device_out=True)
+            rec, rmax = catalog.pack_halos(ctx, res, bdev, cfg.alatt, extended=True, z_max=cfg.z_max)
+            out.records = rec.cpu().numpy()
+            stream.synchronize()
+        out.RTHLmax = rmax
+        out.nhalo = len(out.records)
+        return out
