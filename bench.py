@@ -119,6 +119,33 @@ def read_traffic(path, head, kernel):
     return None
 
 
+def serve_stream(args, ct, tuned):
+    """The c5 serving workload: a Poisson stream of light-cone tiles with
+    nmesh drawn from {128, 256, 384, 512, 768} at the c2 resolution, each
+    synthesised, peak-found, shell-analysed and packed on the GPU; several
+    tiles in flight on separate slots (streams).  Latency includes queueing."""
+    from peakpatch_hip import serve
+    runner = serve.GpuTileRunner(table=ct, tunables=tuned)
+    stream = serve.poisson_stream(args.serve_tiles, args.serve_rate, seed=5)
+    # warm every slot once (plans, pools) with a small tile, untimed
+    slots = [runner.make_slot() for _ in range(args.serve_slots)]
+    for st in slots:
+        runner.run(st, serve.TileRequest(-1, 128, (400.0, 400.0, 400.0), seed=1))
+    with serve.TileServer(slots=len(slots), run_tile=runner.run, slot_states=slots) as srv:
+        res = serve.replay(srv, stream)
+        inflight = srv.max_in_flight
+    for st in slots:
+        runner.release_slot(st)
+    s = serve.summarize(res)
+    s.update({"workload": f"c5: {args.serve_tiles} tiles, Poisson {args.serve_rate:g}/s, nmesh from "
+                          f"{serve.SERVE_NMESH}, alatt 250/256 Mpc/h, 20 filters 1-15 Mpc/h, 2LPT, light-cone",
+              "slots": len(slots), "max_in_flight": inflight,
+              "nmesh_drawn": [r.nmesh for _, r in stream]})
+    if s["failed"]:
+        s["errors"] = sorted({r.error for r in res if r.error})
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +162,10 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="before the timed steps, copy a GIB-sized float32 buffer once (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--serve-tiles", type=int, default=16,
+                    help="tiles in the serving stream (Poisson arrivals, nmesh in {128..768}); 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="tile arrival rate, tiles/s")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_gather_c3_pmc.txt"),
                     help="rocprofv3 --pmc summary whose 'traffic_bytes_per_launch' fills roofline.traffic")
     args = ap.parse_args()
@@ -261,6 +292,9 @@ def main():
         out["plain_peaks_per_s"] = npk / float(np.mean(pw))
         out["speedup_vs_plain_kernel"] = float(np.mean(pk)) / k_s
     out["roofline"]["traffic"] = read_traffic(args.traffic_file, head, dom_name)
+    if args.serve_tiles > 0:
+        out["serving"] = serve_stream(args, ct, tuned)
+        log(f"[bench] serving: {out['serving']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, tile, b, ct, args.cpu_seconds)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")

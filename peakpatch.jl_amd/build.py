@@ -2,6 +2,9 @@
 
     python peakpatch.jl_amd/build.py          # build lib/libpeakpatch_hip.so
 
+PP_BUILD_OUT (output path) and PP_EXTRA_CFLAGS (extra hipcc flags, e.g. the
+tuning macros of pp_post.h) build variants beside the default library.
+
 Every HIP source under csrc/ is compiled to an object (in parallel) and the
 objects are linked with rocFFT into lib/libpeakpatch_hip.so.  All device code
 is built with packed-FP32 instructions disabled.
@@ -16,7 +19,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
-LIB = os.path.join(LIBDIR, "libpeakpatch_hip.so")
+LIB = os.environ.get("PP_BUILD_OUT") or os.path.join(LIBDIR, "libpeakpatch_hip.so")
+OBJDIR = os.path.join(os.path.dirname(LIB), "obj" if "PP_BUILD_OUT" not in os.environ else "obj_" + os.path.basename(LIB))
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PP_OFFLOAD_ARCH", "gfx950")
@@ -26,6 +30,7 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
     f"-I{ROCM}/include",
+    *os.environ.get("PP_EXTRA_CFLAGS", "").split(),
 ]
 
 
@@ -34,7 +39,7 @@ def sources():
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(LIBDIR, "obj", os.path.basename(src) + ".o")
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
     cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -43,7 +48,7 @@ def _compile(src: str) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    os.makedirs(os.path.join(LIBDIR, "obj"), exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))

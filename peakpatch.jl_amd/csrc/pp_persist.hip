@@ -466,16 +466,110 @@ __global__ void __launch_bounds__(64) k_post_wave(const PeakParam *__restrict__ 
 }
 
 // ------------------------------------------------------------ finish
-__global__ void __launch_bounds__(64) k_finish(GatherFields F, const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
-                                               PostConst C, ResultView R)
+// One wave per collapsed wave-gathered peak: Sbar, Sbar2 over the shells
+// inside m0 (shells 1..m0-1), lapd over shells 2..q (cells with r <= R_TH
+// but the centre) and the mask over shells 1..q (RadialShell.jl:756-819),
+// read through the same octant-group batch tables as the gather (rows of
+// neighbouring cells per load instruction), summed in Float64 per lane and
+// reduced by a fixed butterfly.  The reference's d2F stops at the first
+// out-of-bounds cell in r^2 order; a peak whose shells 2..q leave the grid
+// takes the cell-order walk of finish_cells instead.
+struct FinishArgs {
+    GatherFields F;
+    const PeakParam *pps;
+    int64_t lo, hi;
+    const GroupDesc *desc;
+    const int64_t *full_off;
+    const int32_t *full_nb;
+    const int64_t *part_off;
+    const int32_t *part_nb;
+    int cs;
+    PostConst C;
+    ResultView R;
+};
+
+__global__ void __launch_bounds__(64) k_finish(FinishArgs A)
 {
+    const int lane = threadIdx.x;
+    const GatherFields &F = A.F;
+    const PostConst &C = A.C;
+    const ResultView &R = A.R;
     const float *ex[3] = {F.ex, F.ey, F.ez};
     const float *e2[3] = {F.e2x, F.e2y, F.e2z};
-    for (int64_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
-        const PeakParam pk = pps[idx];
+    const long n1 = F.n, n12 = (long)F.n * F.n;
+    const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff;
+    for (int64_t idx = A.lo + blockIdx.x; idx < A.hi; idx += gridDim.x) {
+        const PeakParam pk = A.pps[idx];
         const int64_t o = pk.order;
         if (R.status[o] != DUMP_NONE) continue;
-        finish_cells<true>(C, pk, R.m0[o], R.q[o], ex, e2, R, o);
+        const int m = pk.m, m0 = R.m0[o], q = R.q[o];
+        const int xs = m0 - 1, X = max(xs, q);
+        const bool inside = pk.flags & 1;
+        double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0, sL = 0.0, nL = 0.0;
+        bool oob = false;
+        if (lane == 0) {  // centre cell: shell 1 (in Sbar, in the mask, not in d2F)
+            const long a = pk.ci + pk.cj * n1 + pk.ck * n12;
+            for (int L = 0; L < 3; L++) {
+                Sb[L] += (double)ex[L][a];
+                if (e2[0]) Sb2[L] += (double)e2[L][a];
+            }
+            nS += 1.0;
+            if (C.mask && pk.ci >= lo1 && pk.ci < hi1 && pk.cj >= lo1 && pk.cj < hi1 && pk.ck >= lo1 && pk.ck < hi1)
+                C.mask[a] = 1;
+        }
+        const int nch = X >= 2 ? (X - 2) / A.cs + 1 : 0;
+        for (int c = 0; c < nch; c++) {
+            const int s0 = 2 + c * A.cs;
+            const bool full = s0 + A.cs - 1 <= m;
+            const GroupDesc *tab = A.desc + (full ? A.full_off[c] : A.part_off[m]);
+            const int nb = full ? A.full_nb[c] : A.part_nb[m];
+            for (int b = 0; b < nb; b++) {
+                const GroupDesc g = tab[b * 64 + lane];
+                if (g.s < 2 || g.s > X) continue;
+                const int l = g.lmn & 255, dj = (g.lmn >> 8) & 255, dk = (g.lmn >> 16) & 255;
+                const bool inS = g.s <= xs, inL = g.s <= q;
+#pragma unroll
+                for (int img = 0; img < 8; img++) {
+                    const int sx = (img & 1) ? -1 : 1, sy = (img & 2) ? -1 : 1, sz = (img & 4) ? -1 : 1;
+                    if ((sx < 0 && l == 0) || (sy < 0 && dj == 0) || (sz < 0 && dk == 0)) continue;
+                    const int x = pk.ci + sx * l, y = pk.cj + sy * dj, z = pk.ck + sz * dk;
+                    if (!inside && (x < 0 || x >= F.n || y < 0 || y >= F.n || z < 0 || z >= F.n)) {
+                        oob = oob || inL;
+                        continue;
+                    }
+                    const long a = x + y * n1 + z * n12;
+                    if (inS) {
+                        for (int L = 0; L < 3; L++) {
+                            Sb[L] += (double)ex[L][a];
+                            if (e2[0]) Sb2[L] += (double)e2[L][a];
+                        }
+                        nS += 1.0;
+                    }
+                    if (inL) {
+                        if (C.lapd) { sL += (double)C.lapd[a]; nL += 1.0; }
+                        if (C.mask && x >= lo1 && x < hi1 && y >= lo1 && y < hi1 && z >= lo1 && z < hi1) C.mask[a] = 1;
+                    }
+                }
+            }
+        }
+        if (__ballot(oob) != 0ull) {
+            // rare: R_TH reaches past the grid edge; the ordered walk
+            // recomputes this peak's outputs (its mask writes repeat the same
+            // 1s)
+            finish_cells<true>(C, pk, m0, q, ex, e2, R, o);
+            continue;
+        }
+        for (int L = 0; L < 3; L++) { Sb[L] = wave_sum(Sb[L]); Sb2[L] = wave_sum(Sb2[L]); }
+        nS = wave_sum(nS);
+        sL = wave_sum(sL);
+        nL = wave_sum(nL);
+        if (lane == 0) {
+            for (int L = 0; L < 3; L++) {
+                R.Sbar[o * 3 + L] = nS > 0.0 ? Sb[L] / nS : 0.0;
+                R.Sbar2[o * 3 + L] = nS > 0.0 ? Sb2[L] / nS : 0.0;
+            }
+            R.d2F[o] = (C.lapd && nL > 0.0) ? sL / nL : 0.0;
+        }
     }
 }
 
@@ -846,8 +940,13 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // 6. finish for the wave-gathered peaks
     if (small_lo > 0) {
         const int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * 16));
-        hipLaunchKernelGGL(k_finish, dim3(gf), dim3(64), 0, ctx->stream, G, (const PeakParam *)pps, (int64_t)0, small_lo,
-                           C, S.R);
+        FinishArgs A;
+        A.F = G; A.pps = pps; A.lo = 0; A.hi = small_lo;
+        A.desc = st.d_desc.as<GroupDesc>();
+        A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
+        A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
+        A.cs = cs; A.C = C; A.R = S.R;
+        hipLaunchKernelGGL(k_finish, dim3(gf), dim3(64), 0, ctx->stream, A);
         PP_HIP(hipGetLastError());
     }
     if (cnt) {

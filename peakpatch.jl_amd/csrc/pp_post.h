@@ -18,6 +18,10 @@
 
 #include <cstdint>
 
+#ifndef PP_STRAIN_UNROLL
+#define PP_STRAIN_UNROLL 1  // unroll of the kernel_strain shell loop (overlaps the loads of successive shells)
+#endif
+
 namespace pp {
 
 // Profile field order (per shell): count, F, S(3), S2(3), G(3), Gf(3), SR(9, L*3+K), lapd
@@ -239,6 +243,7 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
     double E[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
     const int4 row = C.wrow[mp - 1];
     const int a = max(mlow, row.y), b = min(mupp, row.z);
+#pragma unroll PP_STRAIN_UNROLL
     for (int m1 = a + X::lane(); m1 <= b; m1 += X::W) {
         const double w = C.wtab[row.x + (m1 - row.y)];
         float G[3], Gf[3], Es[6];

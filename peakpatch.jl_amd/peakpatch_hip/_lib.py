@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpeakpatch_hip.so")
+LIB_PATH = os.environ.get("PEAKPATCH_HIP_LIB") or os.path.join(os.path.dirname(HERE), "lib", "libpeakpatch_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "peakpatch_hip.h")
 
 PP_OK = 0

@@ -18,9 +18,10 @@ def _requests():
             enumerate(sizes)]
 
 
-def test_tiles_same_alone_and_in_mix():
+def test_tiles_same_alone_and_in_mix(homeltab_path):
+    from peakpatch_hip import collapse
     from peakpatch_hip.serve import GpuTileRunner
-    runner = GpuTileRunner(tunables=TUNE)
+    runner = GpuTileRunner(table=collapse.read_homeltab(homeltab_path), tunables=TUNE)
     # alone: one slot, each tile submitted after the previous one finished
     st = [runner.make_slot()]
     with TileServer(slots=1, run_tile=runner.run, slot_states=st) as srv:
