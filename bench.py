@@ -160,8 +160,8 @@ def main():
     ap.add_argument("--tune", default="", help="pin launch parameters, e.g. small_m=12,chunk_shells=128 "
                     "(implies --no-autotune)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
-                    help="before the timed steps, copy a GIB-sized float32 buffer once (a kernel of known "
-                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+                    help="before the timed steps, run one elementwise kernel over a GIB-sized float32 buffer "
+                         "(known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--serve-tiles", type=int, default=16,
                     help="tiles in the serving stream (Poisson arrivals, nmesh in {128..768}); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="tile arrival rate, tiles/s")
@@ -218,9 +218,9 @@ def main():
         src = torch.ones(nel, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)  # an elementwise kernel (a same-type copy_ may go to the DMA engine)
         torch.cuda.synchronize()
-        log(f"[bench] copy calibration: {nel * 4} bytes read, {nel * 4} bytes written")
+        log(f"[bench] copy calibration: elementwise add, {nel * 4} bytes read, {nel * 4} bytes written")
         del src, dst
 
     forms = []

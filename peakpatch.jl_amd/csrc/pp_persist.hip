@@ -204,19 +204,30 @@ __device__ __forceinline__ double wave_scan(double x)
 }
 
 // ------------------------------------------------------- small peaks
-// One segment of small peaks with equal m, in sorted order [lo, hi); its
-// blocks of P peaks are numbered from blk0.
+// Small peaks (m <= small_m) are read cell by cell: the cells of shells 1..m
+// in row-major order, so each load instruction covers whole x-runs of the
+// peaks' rows.  A block handles P = floor(64 / m) peaks of equal m; the host
+// cut their P x ncells cells into dense 64-lane batches and sorted every
+// batch by (peak, shell).  Each lane stages its cell's 21 contributions in LDS and the
+// first lane of each (peak, shell) run adds the run in lane order to that
+// shell's LDS bin.  Records carry S, S2 and lapd so the lane post can finish
+// these peaks itself.
+struct CellSlot {
+    int32_t cell;   // (di+128) | (dj+128) << 8 | (dk+128) << 16; -1 for an empty lane
+    int16_t q, s;   // peak slot in the block, shell (1-based)
+    int32_t run;    // length of the (q, s) run starting here, 0 if not its first lane
+    int32_t pad;
+};
+
 struct SmallSeg {
-    int32_t m, G, P, doff;  // shells, groups, peaks per wave, first group descriptor
-    int32_t soff, pad;      // first shell boundary (groups of shell s: [segs[soff+s-2], segs[soff+s-1]))
-    int64_t lo, hi, blk0;
+    int32_t m, P, nbat, toff;  // shells, peaks per block, batches per block, first CellSlot
+    int64_t lo, hi, blk0;      // sorted peaks [lo, hi); first block
 };
 
 struct SmallArgs {
     GatherFields F;
     const PeakParam *pps;
-    const GroupDesc *desc;   // groups of shells 2..m, sorted by shell
-    const int32_t *segs;
+    const CellSlot *tab;
     const SmallSeg *seg;
     int nseg;
     int64_t nblk;
@@ -225,30 +236,69 @@ struct SmallArgs {
     const double *rad;
 };
 
-// P * m <= 64 for every small m (P = floor(64 / groups)), so one pass of the
-// 64 lanes writes all records of a block.
+constexpr int NSF = 21;  // staged fields: record slots 0..13 and 16..22
+__device__ __forceinline__ constexpr int sf_slot(int f) { return f < P_NB ? f : f + 2; }
+
 __global__ void __launch_bounds__(64) k_small(SmallArgs A)
 {
-    __shared__ float part[PR][64];
+    __shared__ float stg[NSF][64];
+    __shared__ float bins[NSF][64];  // (peak q, shell s) at q m + s - 1
     __shared__ double sdF[64], sfb[64];
     const int lane = threadIdx.x;
+    const GatherFields &F = A.F;
+    const long n1 = F.n, n12 = (long)F.n * F.n;
     for (int64_t blk = blockIdx.x; blk < A.nblk; blk += gridDim.x) {
         int si = 0;
         while (si + 1 < A.nseg && blk >= A.seg[si + 1].blk0) si++;
         const SmallSeg S = A.seg[si];
-        const int m = S.m, G = S.G, P = S.P;
+        const int m = S.m, P = S.P;
         const int64_t pbase = S.lo + (blk - S.blk0) * P;
-        float acc[PR];
-#pragma unroll
-        for (int f = 0; f < PR; f++) acc[f] = 0.0f;
-        if (G > 0) {
-            const int q = lane / G, g = lane - q * G;
-            const int64_t idx = pbase + q;
-            if (q < P && idx < S.hi) group_sum<true>(A.F, A.pps[idx], A.desc[S.doff + g], acc);
-        }
-#pragma unroll
-        for (int f = 0; f < PR; f++) part[f][lane] = acc[f];
+        for (int f = 0; f < NSF; f++) bins[f][lane] = 0.0f;
         __syncthreads();
+        for (int b = 0; b < S.nbat; b++) {
+            const CellSlot t = A.tab[S.toff + b * 64 + lane];
+            const int64_t idx = pbase + t.q;
+            const bool live = t.cell >= 0 && idx < S.hi;
+            float acc[NSF];
+#pragma unroll
+            for (int f = 0; f < NSF; f++) acc[f] = 0.0f;
+            if (live) {
+                const PeakParam pk = A.pps[idx];
+                const int dx = (t.cell & 255) - 128, dy = ((t.cell >> 8) & 255) - 128, dz = ((t.cell >> 16) & 255) - 128;
+                const int x = pk.ci + dx, y = pk.cj + dy, z = pk.ck + dz;
+                if ((pk.flags & 1) || (x >= 0 && x < F.n && y >= 0 && y < F.n && z >= 0 && z < F.n)) {
+                    const long a = x + y * n1 + z * n12;
+                    const float d = F.d[a], dT = F.dT[a], ex = F.ex[a], ey = F.ey[a], ez = F.ez[a];
+                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                    acc[P_G + 0] = d * fx; acc[P_G + 1] = d * fy; acc[P_G + 2] = d * fz;
+                    acc[P_GF + 0] = dT * fx; acc[P_GF + 1] = dT * fy; acc[P_GF + 2] = dT * fz;
+                    acc[P_E + 0] = 2.0f * (ex * fx);
+                    acc[P_E + 1] = 2.0f * (ey * fy);
+                    acc[P_E + 2] = 2.0f * (ez * fz);
+                    acc[P_E + 3] = ex * fy + ey * fx;
+                    acc[P_E + 4] = ex * fz + ez * fx;
+                    acc[P_E + 5] = ey * fz + ez * fy;
+                    acc[P_F] = d;
+                    acc[P_N] = 1.0f;
+                    acc[P_NB + 0] = ex; acc[P_NB + 1] = ey; acc[P_NB + 2] = ez;
+                    if (F.e2x) { acc[P_NB + 3] = F.e2x[a]; acc[P_NB + 4] = F.e2y[a]; acc[P_NB + 5] = F.e2z[a]; }
+                    if (F.lap) acc[P_NB + 6] = F.lap[a];
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < NSF; f++) stg[f][lane] = acc[f];
+            __syncthreads();
+            if (live && t.run > 0) {
+                const int o = t.q * m + t.s - 1;
+#pragma unroll
+                for (int f = 0; f < NSF; f++) {
+                    float v = stg[f][lane];
+                    for (int r = 1; r < t.run; r++) v += stg[f][lane + r];
+                    bins[f][o] += v;
+                }
+            }
+            __syncthreads();
+        }
         // record of (peak q, shell s) on lane o = q m + s - 1
         const int q = lane / m, s = 1 + (lane - q * m);
         const int64_t idx = pbase + q;
@@ -257,15 +307,9 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
 #pragma unroll
         for (int f = 0; f < PR; f++) v[f] = 0.0f;
         if (act) {
-            if (s == 1) {
-                center_values<true>(A.F, A.pps[idx], v);
-            } else {
-                const int g0 = q * G + A.segs[S.soff + s - 2], g1 = q * G + A.segs[S.soff + s - 1];
-                for (int gg = g0; gg < g1; gg++) {
 #pragma unroll
-                    for (int f = 0; f < PR; f++) v[f] += part[f][gg];
-                }
-            }
+            for (int f = 0; f < NSF; f++) v[sf_slot(f)] = bins[f][lane];
+            if (s == 1) v[P_L] = 0.0f;  // d2F starts at cell 2
         }
         sdF[lane] = v[P_N] > 0.0f ? (double)v[P_F] / (double)v[P_N] : 0.0;
         __syncthreads();
@@ -437,8 +481,12 @@ struct AosProfile {
     }
 };
 
+#ifndef PP_POST_LANE_WAVES
+#define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
+#endif
+
 template <bool FULL>
-__global__ void __launch_bounds__(64) k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST_LANE_WAVES))) k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
                                                   float *__restrict__ prof, const int64_t *__restrict__ prof_off,
                                                   PostConst C, ResultView R, int has_lap)
 {
@@ -504,6 +552,7 @@ __global__ void __launch_bounds__(64) k_finish(FinishArgs A)
         if (R.status[o] != DUMP_NONE) continue;
         const int m = pk.m, m0 = R.m0[o], q = R.q[o];
         const int xs = m0 - 1, X = max(xs, q);
+        const int r2X = X >= 1 ? C.shell_r2[X - 1] : 0;
         const bool inside = pk.flags & 1;
         double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0, sL = 0.0, nL = 0.0;
         bool oob = false;
@@ -525,8 +574,13 @@ __global__ void __launch_bounds__(64) k_finish(FinishArgs A)
             const int nb = full ? A.full_nb[c] : A.part_nb[m];
             for (int b = 0; b < nb; b++) {
                 const GroupDesc g = tab[b * 64 + lane];
-                if (g.s < 2 || g.s > X) continue;
                 const int l = g.lmn & 255, dj = (g.lmn >> 8) & 255, dk = (g.lmn >> 16) & 255;
+                // rows come by increasing dj^2 + dk^2: once a batch's smallest
+                // row offset is beyond shell X, so is every later batch
+                int amin = g.s >= 2 ? dj * dj + dk * dk : 0x7fffffff;
+                for (int o2 = 32; o2 >= 1; o2 >>= 1) amin = min(amin, __shfl_xor(amin, o2, 64));
+                if (amin > r2X) break;
+                if (g.s < 2 || g.s > X) continue;
                 const bool inS = g.s <= xs, inL = g.s <= q;
 #pragma unroll
                 for (int img = 0; img < 8; img++) {
@@ -575,7 +629,7 @@ __global__ void __launch_bounds__(64) k_finish(FinishArgs A)
 
 // ------------------------------------------------------------- host side
 struct SmallTab {
-    int32_t doff = 0, soff = 0, G = 0;
+    int32_t toff = 0, P = 0, nbat = 0;
 };
 
 struct PersistState {
@@ -584,12 +638,11 @@ struct PersistState {
     std::vector<GroupDesc> desc;                      // chunk tables
     std::vector<std::pair<int64_t, int>> full;        // per chunk: (offset, batches)
     std::map<int, std::pair<int64_t, int>> part;      // per m: last partial chunk
-    std::vector<GroupDesc> sdesc;                     // small tables
-    std::vector<int32_t> ssegs;
+    std::vector<CellSlot> scells;                     // small-peak batch tables
     std::vector<SmallTab> stab;                       // index m (0..small_max)
     int small_max = 0;
     std::vector<SmallSeg> segs_host;
-    DevBuf d_desc, d_full_off, d_full_nb, d_part_off, d_part_nb, d_sdesc, d_ssegs, d_seg;
+    DevBuf d_desc, d_full_off, d_full_nb, d_part_off, d_part_nb, d_scells, d_seg;
     bool dirty = true;
 };
 
@@ -608,8 +661,8 @@ void persist_release(pp_ctx *ctx)
     auto it = g_state.find(ctx);
     if (it == g_state.end()) return;
     PersistState &st = it->second;
-    DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb,
-                      &st.d_sdesc, &st.d_ssegs, &st.d_seg};
+    DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb, &st.d_scells,
+                      &st.d_seg};
     for (DevBuf *b : bufs) b->release();
     g_state.erase(it);
 }
@@ -621,26 +674,32 @@ static void for_groups(const ShellTable &T, const std::vector<int32_t> &shell_of
 {
     const int r2lo = T.shell_r2[slo - 1], r2hi = T.shell_r2[shi - 1];
     const int R = (int)std::floor(std::sqrt((double)r2hi));
+    // rows (dj, dk) by increasing a = dj^2 + dk^2, so a batch's smallest a
+    // bounds every later batch's radii (the finish pass stops there); each
+    // row's l-run stays contiguous, which is what the loads coalesce on
+    std::vector<std::pair<int, int>> rows;
     for (int dk = 0; dk <= R; dk++)
-        for (int dj = 0; dj <= R; dj++) {
-            const int a = dk * dk + dj * dj;
-            if (a > r2hi) continue;
-            int l = 0;
-            if (r2lo - a > 0) {
-                l = (int)std::floor(std::sqrt((double)(r2lo - a)));
-                while (l * l + a < r2lo) l++;
-                while (l > 0 && (l - 1) * (l - 1) + a >= r2lo) l--;
-            }
-            for (; l * l + a <= r2hi; l++) {
-                const int r2 = l * l + a;
-                if (r2 == 0) continue;
-                fn(GroupDesc{l | (dj << 8) | (dk << 16), shell_of_r2[r2]});
-            }
+        for (int dj = 0; dj <= R; dj++)
+            if (dk * dk + dj * dj <= r2hi) rows.push_back({dk * dk + dj * dj, dk * 1024 + dj});
+    std::stable_sort(rows.begin(), rows.end());
+    for (const auto &rw : rows) {
+        const int a = rw.first, dk = rw.second / 1024, dj = rw.second % 1024;
+        int l = 0;
+        if (r2lo - a > 0) {
+            l = (int)std::floor(std::sqrt((double)(r2lo - a)));
+            while (l * l + a < r2lo) l++;
+            while (l > 0 && (l - 1) * (l - 1) + a >= r2lo) l--;
         }
+        for (; l * l + a <= r2hi; l++) {
+            const int r2 = l * l + a;
+            if (r2 == 0) continue;
+            fn(GroupDesc{l | (dj << 8) | (dk << 16), shell_of_r2[r2]});
+        }
+    }
 }
 
-// Groups of shells [slo, shi] in row-major order cut into consecutive
-// 64-slot batches (neighbouring x-runs share cache lines); inside a batch the
+// Groups of shells [slo, shi] in row order cut into consecutive 64-slot
+// batches (neighbouring x-runs share cache lines); inside a batch the
 // groups are sorted by shell and the first lane of each run of equal shells
 // carries the run length in lmn bits 24-30 (its deposit lane).
 static void pack_groups(const ShellTable &T, const std::vector<int32_t> &shell_of_r2, int slo, int shi,
@@ -672,41 +731,67 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
     PersistState &st = state_of(ctx);
     const ShellTable &T = ctx->shells;
     if (st.rmax != T.rmax || st.cs != cs) {
-        DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb,
-                          &st.d_sdesc, &st.d_ssegs, &st.d_seg};
+        DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb, &st.d_scells,
+                          &st.d_seg};
         for (DevBuf *b : bufs) b->release();
         st = PersistState();
         st.rmax = T.rmax;
         st.cs = cs;
         st.shell_of_r2.assign((size_t)T.shell_r2[T.nshells - 1] + 1, 0);
         for (int s = 1; s <= T.nshells; s++) st.shell_of_r2[T.shell_r2[s - 1]] = s;
-        // small tables: groups of shells 2..m sorted by shell
+        // small-peak tables: cells of shells 1..m in row-major order (dk, dj,
+        // di), P peaks per block, cut into 64-lane batches sorted by (peak,
+        // shell) with the run length on each run's first lane
         st.stab.assign(SMALL_MAX + 1, SmallTab());
-        st.small_max = 1;
-        for (int m = 2; m <= std::min(SMALL_MAX, (int)T.nshells); m++) {
-            std::vector<GroupDesc> g;
-            for_groups(T, st.shell_of_r2, 2, m, [&](GroupDesc d) { g.push_back(d); });
-            if (g.size() > 64) break;
-            std::stable_sort(g.begin(), g.end(), [](const GroupDesc &a, const GroupDesc &b) { return a.s < b.s; });
+        st.small_max = std::min(SMALL_MAX, (int)T.nshells);
+        for (int m = 1; m <= st.small_max; m++) {
+            const int r2m = T.shell_r2[m - 1];
+            const int R = (int)std::floor(std::sqrt((double)r2m));
+            std::vector<std::pair<int32_t, int16_t>> cells;  // (packed cell, shell)
+            for (int dk = -R; dk <= R; dk++)
+                for (int dj = -R; dj <= R; dj++)
+                    for (int di = -R; di <= R; di++) {
+                        const int r2 = di * di + dj * dj + dk * dk;
+                        if (r2 > r2m) continue;
+                        cells.push_back({(di + 128) | ((dj + 128) << 8) | ((dk + 128) << 16),
+                                         (int16_t)st.shell_of_r2[r2]});
+                    }
+            const int nc = (int)cells.size();
+            const int P = std::max(1, 64 / m);  // the P m records of a block fit its 64 bins
+            const int nv = P * nc, nbat = (nv + 63) / 64;
             SmallTab t;
-            t.doff = (int32_t)st.sdesc.size();
-            t.soff = (int32_t)st.ssegs.size();
-            t.G = (int32_t)g.size();
-            size_t q = 0;
-            for (int s = 2; s <= m + 1; s++) {
-                while (q < g.size() && g[q].s < s) q++;
-                st.ssegs.push_back((int32_t)q);
+            t.toff = (int32_t)st.scells.size();
+            t.P = P;
+            t.nbat = nbat;
+            for (int b = 0; b < nbat; b++) {
+                std::vector<CellSlot> bt;
+                for (int v = b * 64; v < std::min(nv, b * 64 + 64); v++) {
+                    CellSlot c;
+                    c.cell = cells[v % nc].first;
+                    c.q = (int16_t)(v / nc);
+                    c.s = cells[v % nc].second;
+                    c.run = 0;
+                    c.pad = 0;
+                    bt.push_back(c);
+                }
+                std::stable_sort(bt.begin(), bt.end(), [](const CellSlot &x, const CellSlot &y) {
+                    return x.q != y.q ? x.q < y.q : x.s < y.s;
+                });
+                for (size_t q0 = 0; q0 < bt.size();) {
+                    size_t r = q0;
+                    while (r < bt.size() && bt[r].q == bt[q0].q && bt[r].s == bt[q0].s) r++;
+                    bt[q0].run = (int32_t)(r - q0);
+                    q0 = r;
+                }
+                while (bt.size() < 64) bt.push_back(CellSlot{-1, 0, 1, 0, 0});
+                st.scells.insert(st.scells.end(), bt.begin(), bt.end());
             }
-            st.sdesc.insert(st.sdesc.end(), g.begin(), g.end());
             st.stab[m] = t;
-            st.small_max = m;
         }
-        PP_CHECK(st.d_sdesc.grow(std::max<size_t>(1, st.sdesc.size()) * sizeof(GroupDesc)));
-        PP_CHECK(st.d_ssegs.grow(std::max<size_t>(1, st.ssegs.size()) * sizeof(int32_t)));
-        if (!st.sdesc.empty())
-            PP_HIP(hipMemcpy(st.d_sdesc.ptr, st.sdesc.data(), st.sdesc.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
-        if (!st.ssegs.empty())
-            PP_HIP(hipMemcpy(st.d_ssegs.ptr, st.ssegs.data(), st.ssegs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        PP_CHECK(st.d_scells.grow(std::max<size_t>(1, st.scells.size()) * sizeof(CellSlot)));
+        if (!st.scells.empty())
+            PP_HIP(hipMemcpy(st.d_scells.ptr, st.scells.data(), st.scells.size() * sizeof(CellSlot),
+                             hipMemcpyHostToDevice));
     }
     for (int m : ms) {
         if (m < 2) continue;
@@ -868,10 +953,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         if (hi <= lo) continue;
         SmallSeg sg{};
         sg.m = m;
-        sg.G = st.stab[m].G;
-        sg.P = std::min(sg.G > 0 ? 64 / sg.G : 64, 64 / m);  // k_small writes the P * m records in one pass
-        sg.doff = st.stab[m].doff;
-        sg.soff = st.stab[m].soff;
+        sg.P = st.stab[m].P;
+        sg.nbat = st.stab[m].nbat;
+        sg.toff = st.stab[m].toff;
         sg.lo = lo;
         sg.hi = hi;
         sg.blk0 = nblk;
@@ -906,7 +990,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     if (nblk > 0) {
         SmallArgs A;
-        A.F = G; A.pps = pps; A.desc = st.d_sdesc.as<GroupDesc>(); A.segs = st.d_ssegs.as<int32_t>();
+        A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
         A.seg = st.d_seg.as<SmallSeg>(); A.nseg = (int)st.segs_host.size(); A.nblk = nblk;
         A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));

@@ -71,6 +71,7 @@ struct PostConst {
     const int32_t *cell;  // packed offsets, r^2 sorted
     const int32_t *shell_start;
     const int32_t *shell_ge; // [r2max + 2]: first shell (1-based) with r2 >= k
+    const int32_t *shell_r2; // r2 of shell s at [s - 1]
     int r2max, nshells;
     Table3 tab;
     double wRnor, aRnor;

@@ -348,6 +348,7 @@ int prepare_post_const(pp_ctx *ctx, int n, int nbuff, double alatt, const float 
     C.akk = T.d_akk.as<double>();
     C.wrow = T.d_wrow.as<int4>();
     C.shell_ge = T.d_shell_ge.as<int32_t>();
+    C.shell_r2 = T.d_shell_r2.as<int32_t>();
     C.r2max = T.shell_r2[T.nshells - 1];
     C.nshells = T.nshells;
     C.wtab = T.d_wtab.as<double>();

@@ -99,6 +99,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not built; run `python peakpatch.jl_amd/build.py`")
+    # PyTorch-ROCm ships its own HIP runtime.  Whichever copy loads first
+    # serves the process (same SONAME), and torch cannot start its GPU on a
+    # runtime loaded before it, so torch, when present, is loaded first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -47,7 +47,8 @@ def test_pack_halos_bitexact(ctx, extended):
     ref, rref = OC.pack(hres, host["x"], host["y"], host["z"], host["Rf"], host["ZZon"], host["D"], host["v2fac"],
                         host["FcRf"], host["d2Rf"], 1.17, extended)
     assert len(ref) > 20
-    assert np.array_equal(rec.cpu().numpy(), ref)
+    # gradpkrf is NaN where the reference divides by rad[1] = 0 (NaN == NaN here)
+    assert np.array_equal(rec.cpu().numpy(), ref, equal_nan=True)
     assert rmax == rref
 
 
@@ -59,7 +60,7 @@ def test_pack_halos_zmax_cut_and_capacity(ctx):
     ref, rref = OC.pack(hres, host["x"], host["y"], host["z"], host["Rf"], host["ZZon"], host["D"], host["v2fac"],
                         host["FcRf"], host["d2Rf"], 1.0, False, z_max=zcut)
     assert 0 < len(ref) < int((hres["RTHL"] > 0).sum())
-    assert np.array_equal(rec.cpu().numpy(), ref) and rmax == rref
+    assert np.array_equal(rec.cpu().numpy(), ref, equal_nan=True) and rmax == rref
     # a buffer too small is an error, not a truncation
     import ctypes as C
     from peakpatch_hip import _lib as L

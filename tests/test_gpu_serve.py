@@ -39,7 +39,8 @@ def test_tiles_same_alone_and_in_mix(homeltab_path):
     assert sum(r.nhalo for r in alone) > 50
     for a, m in zip(alone, mix):
         assert a.tile_id == m.tile_id and a.npeaks == m.npeaks and a.nhalo == m.nhalo
-        assert np.array_equal(a.records, m.records), f"tile {a.tile_id} differs in the mix"
+        # (gradpkrf is NaN where the reference divides by rad[1] = 0)
+        assert np.array_equal(a.records, m.records, equal_nan=True), f"tile {a.tile_id} differs in the mix"
         assert a.RTHLmax == m.RTHLmax
     s = summarize(mix)
     assert s["completed"] == 6 and s["p99_ms"] >= s["p50_ms"] > 0
