@@ -134,11 +134,13 @@ struct pp_ctx {
     // persistent-form tunables (pp_autotune_shell): small-peak threshold in
     // finalised shells (several peaks per wave), shells per LDS chunk of the
     // wave gather, the shell count above which a peak's Float64 post runs on
-    // a whole wave instead of one lane, and the cap on gather waves per CU
+    // a whole wave, the cap on gather waves per CU, and the lanes per peak of
+    // the post below that count
     int tiny_m = 16;
     int chunk_shells = 64;
     int coop_m = 64;
     int gather_waves_cu = 16;
+    int post_width = 1;  // lanes per peak of the post below coop_m (1, 4, 8, 16)
     // events: [0,1] whole call, [2..8] persistent-form stage boundaries
     static constexpr int NEV = 9;
     hipEvent_t ev[NEV] = {};

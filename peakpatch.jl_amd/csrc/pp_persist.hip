@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST
         const PeakParam pk = pps[idx];
         float *rec = prof + prof_off[idx] * PR;
         AosProfile<FULL> P{rec, C.rad, has_lap != 0};
-        analyse_post<false, FULL>(P, pk, C, R, pk.order);
+        analyse_post<1, FULL>(P, pk, C, R, pk.order);
         R.nshells[pk.order] = pk.m;
     }
 }
@@ -508,8 +508,27 @@ __global__ void __launch_bounds__(64) k_post_wave(const PeakParam *__restrict__ 
         const PeakParam pk = pps[idx];
         float *rec = prof + prof_off[idx] * PR;
         AosProfile<false> P{rec, C.rad, false};
-        analyse_post<true, false>(P, pk, C, R, pk.order);
+        analyse_post<64, false>(P, pk, C, R, pk.order);
         if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
+    }
+}
+
+// W lanes per peak, 64 / W peaks per wave
+template <int W>
+__global__ void __launch_bounds__(64) k_post_group(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
+                                                   float *__restrict__ prof, const int64_t *__restrict__ prof_off,
+                                                   PostConst C, ResultView R)
+{
+    constexpr int G = 64 / W;
+    const int grp = threadIdx.x / W;
+    for (int64_t b0 = lo + (int64_t)blockIdx.x * G; b0 < hi; b0 += (int64_t)gridDim.x * G) {
+        const int64_t idx = b0 + grp;
+        if (idx >= hi) continue;  // whole lane groups drop out
+        const PeakParam pk = pps[idx];
+        float *rec = prof + prof_off[idx] * PR;
+        AosProfile<false> P{rec, C.rad, false};
+        analyse_post<W, false>(P, pk, C, R, pk.order);
+        if ((threadIdx.x & (W - 1)) == 0) R.nshells[pk.order] = pk.m;
     }
 }
 
@@ -1018,7 +1037,21 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         return PP_OK;
     };
     PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
-    PP_CHECK(post_lane(coop_end, small_lo, false));
+    if (ctx->post_width <= 1) {
+        PP_CHECK(post_lane(coop_end, small_lo, false));
+    } else if (small_lo > coop_end) {
+        const int W = ctx->post_width;
+        const int64_t nwav = (small_lo - coop_end + 64 / W - 1) / (64 / W);
+        const int gp = (int)std::max<int64_t>(1, std::min<int64_t>(nwav, (int64_t)ctx->num_cu * 16));
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, coop_end, small_lo,
+                               d_rec, (const int64_t *)d_off, C, S.R);
+        };
+        if (W == 4) launch(k_post_group<4>);
+        else if (W == 8) launch(k_post_group<8>);
+        else launch(k_post_group<16>);
+        PP_HIP(hipGetLastError());
+    }
     PP_CHECK(post_lane(small_lo, np, true));
     PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     // 6. finish for the wave-gathered peaks

@@ -218,18 +218,30 @@ struct Strain {
     double g[3], gf[3];
 };
 
-// Execution modes of the post-process: COOP = the 64 lanes of a wave work on
-// one peak (radial sums split over lanes, butterfly-combined); !COOP = every
-// lane runs its own peak serially.
-template <bool COOP> struct Exec {
-    static constexpr int W = COOP ? 64 : 1;
-    __device__ static int lane() { return COOP ? (int)(threadIdx.x & 63) : 0; }
+// Execution widths of the post-process: W lanes work on one peak (radial
+// sums split over the W lanes of an aligned lane group, combined by a fixed
+// butterfly); W = 1: every lane runs its own peak serially, W = 64: one peak
+// per wave.  All lanes of a group take the same branches (they hold the same
+// peak), so the group's shuffles and ballots see all of its lanes.
+template <int W> struct Exec {
+    static_assert(W == 1 || W == 4 || W == 8 || W == 16 || W == 32 || W == 64, "lane-group width");
+    static constexpr bool COOP = W > 1;
+    __device__ static int lane() { return W == 1 ? 0 : (int)(threadIdx.x & (W - 1)); }
+    __device__ static int base() { return (int)(threadIdx.x & 63) & ~(W - 1); }
     __device__ static double sum(double v)
     {
-        if constexpr (COOP) return wave_sum(v);
-        else return v;
+#pragma unroll
+        for (int o = W / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        return v;
     }
-    __device__ static bool leader() { return COOP ? (threadIdx.x & 63) == 0 : true; }
+    // bits of the group's lanes (bit i: lane i of the group)
+    __device__ static unsigned long long ballot(bool hit)
+    {
+        const unsigned long long b = __ballot(hit);
+        if constexpr (W == 64) return b;
+        else return (b >> base()) & ((1ull << W) - 1);
+    }
+    __device__ static bool leader() { return lane() == 0; }
 };
 
 // kernel_strain (RadialShell.jl:228-277).  The weight of shell m1 around
@@ -237,15 +249,15 @@ template <bool COOP> struct Exec {
 // per shell table (k_wtab); the row of mp holds exactly the shells with
 // (rad(mp) - rad(m1))^2 < 4, the reference's cut, so clamping [mlow, mupp]
 // to the row is the reference's skip.
-template <bool COOP, class PV>
+template <int W, class PV>
 __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlow, int mupp)
 {
-    using X = Exec<COOP>;
+    using X = Exec<W>;
     double E[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
     const int4 row = C.wrow[mp - 1];
     const int a = max(mlow, row.y), b = min(mupp, row.z);
 #pragma unroll PP_STRAIN_UNROLL
-    for (int m1 = a + X::lane(); m1 <= b; m1 += X::W) {
+    for (int m1 = a + X::lane(); m1 <= b; m1 += W) {
         const double w = C.wtab[row.x + (m1 - row.y)];
         float G[3], Gf[3], Es[6];
         P.strain_raw(m1, G, Gf, Es);
@@ -339,7 +351,7 @@ template <bool COOP>
 __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, int q,
                              const float *const ex[3], const float *const e2[3], const ResultView &R, int64_t o)
 {
-    using X = Exec<COOP>;
+    using X = Exec<COOP ? 64 : 1>;
     const int cS = C.shell_start[m0 - 1];          // cells of shells 1..m0-1
     const int cend = min(C.shell_start[q], pk.npart);
     const long n1 = C.n1, n12 = (long)C.n1 * C.n2;
@@ -350,7 +362,7 @@ __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, in
     const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
     const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
     const int cmax = max(cS, cend);
-    for (int c = X::lane(); c < cmax; c += X::W) {
+    for (int c = X::lane(); c < cmax; c += (COOP ? 64 : 1)) {
         const int pc = C.cell[c];
         const int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
         const bool inb = inside || (x >= 0 && x < C.n1 && y >= 0 && y < C.n2 && z >= 0 && z < C.n3);
@@ -384,7 +396,7 @@ __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, in
         } else {
             // stop at the first out-of-bounds cell: recount cells [1, firstoob)
             sL = 0.0; nL = 0.0;
-            for (int c = 1 + X::lane(); c < firstoob; c += X::W) {
+            for (int c = 1 + X::lane(); c < firstoob; c += (COOP ? 64 : 1)) {
                 const int pc = C.cell[c];
                 const int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
                 sL += (double)C.lapd[x + y * n1 + z * n12];
@@ -408,10 +420,10 @@ __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, in
 // count(s) and strain_inputs(s); with FINISH it also provides S / S2 / lap
 // sums per shell and computes Sbar, Sbar2, d2F and the mask here; without
 // FINISH those are left to finish_cells() and the final m0 is recorded.
-template <bool COOP, bool FINISH, class PV>
+template <int W, bool FINISH, class PV>
 __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C, const ResultView &R, int64_t o)
 {
-    using X = Exec<COOP>;
+    using X = Exec<W>;
     const int lane = X::lane();
     const int m = pk.m;
     int status = DUMP_NONE;
@@ -425,7 +437,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         if (mrf > 1) { int a = first_rad_gt(C, 1, mrf - 1, rlow_rf); if (a <= mrf - 1) mlow_rf = a; }
         int mupp_rf = mrf;
         { int b = last_rad_lt(C, mrf + 1, m, P.rad(mrf) + 2.0); if (b >= mrf + 1) mupp_rf = b; }
-        Strain srf = kernel_strain<COOP>(P, C, mrf, mlow_rf, mupp_rf);
+        Strain srf = kernel_strain<W>(P, C, mrf, mlow_rf, mupp_rf);
         const double gradpkrf[3] = {srf.g[0], srf.g[1], srf.g[2]};
 
         // ---- Phase 4: fcrit crossing (RadialShell.jl:492-538) ----
@@ -433,11 +445,11 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         int m0 = pk.m0, mupp;
         if (P.Fbar(m0) >= fcrit) {
             int found = 0;
-            if constexpr (COOP) {
-                for (int base = m0; base <= m; base += 64) {
+            if constexpr (W > 1) {
+                for (int base = m0; base <= m; base += W) {
                     int mm = base + lane;
                     bool hit = mm <= m && P.Fbar(mm) < fcrit;
-                    unsigned long long b = __ballot(hit);
+                    unsigned long long b = X::ballot(hit);
                     if (b) { m0 = base + __ffsll((long long)b) - 1; found = 1; break; }
                 }
             } else {
@@ -452,11 +464,11 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         } else {
             int mstart = m0 - 1 > 1 ? m0 - 1 : 1;
             int found = 0;
-            if constexpr (COOP) {
-                for (int top = mstart; top >= 1; top -= 64) {
+            if constexpr (W > 1) {
+                for (int top = mstart; top >= 1; top -= W) {
                     int mp = top - lane;
                     bool hit = mp >= 1 && P.Fbar(mp) >= fcrit;
-                    unsigned long long b = __ballot(hit);
+                    unsigned long long b = X::ballot(hit);
                     if (b) { m0 = top - (__ffsll((long long)b) - 1) + 1; found = 1; break; }
                 }
             } else {
@@ -475,7 +487,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         int mlow = m0;
         if (m0 > 1) { int a = first_rad_gt(C, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
         const double ZZon = pk.ZZon;
-        Strain s0 = kernel_strain<COOP>(P, C, m0, mlow, mupp);
+        Strain s0 = kernel_strain<W>(P, C, m0, mlow, mupp);
         if (!finite6(s0.E)) { status = DUMP_NONFINITE; goto nocol; }
         double Frho_m0, e_v_m0, p_v_m0;
         evals_ep(s0.E, Frho_m0, e_v_m0, p_v_m0);
@@ -503,7 +515,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     int a = first_rad_gt(C, 1, mlow - 1, rlow_mp);
                     mlow = a <= mlow - 1 ? a : mlow;
                 }
-                Strain smp = kernel_strain<COOP>(P, C, mp, mlow, mupp);
+                Strain smp = kernel_strain<W>(P, C, mp, mlow, mupp);
                 if (!finite6(smp.E)) { status = DUMP_NONFINITE; goto nocol; }
                 double Frho_mp, e_v_mp, p_v_mp;
                 evals_ep(smp.E, Frho_mp, e_v_mp, p_v_mp);
@@ -574,7 +586,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
 
         // ---- Phase 7 (RadialShell.jl:756-819) ----
         double Srb = 0.0;
-        for (int m1 = 2 + lane; m1 <= m0 - 1; m1 += X::W) {
+        for (int m1 = 2 + lane; m1 <= m0 - 1; m1 += W) {
             double ra = P.rad(m1 - 1), rb = P.rad(m1);
             double r5a = m1 == 2 ? 0.0 : ra * ra * ra * ra * ra, r5b = rb * rb * rb * rb * rb;
             Srb += 0.5 * (P.Fbar(m1 - 1) + P.Fbar(m1)) * (r5b - r5a);
@@ -591,7 +603,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
 
         if constexpr (FINISH) {
             double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
-            for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += X::W) {
+            for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += W) {
                 double cnt = P.count(m1);
                 if (cnt > 0.0) {
                     for (int L = 0; L < 3; L++) { Sb[L] += P.S(m1, L); Sb2[L] += P.S2(m1, L); }
@@ -608,7 +620,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
             if (C.mask) {
                 const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
                 const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
-                for (int c = lane; c < cend; c += X::W) {
+                for (int c = lane; c < cend; c += W) {
                     int pc = C.cell[c];
                     int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
                     if (x < lo1 || x >= hi1 || y < lo2 || y >= hi2 || z < lo3 || z >= hi3) continue;
@@ -619,7 +631,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
             if (P.has_lapd()) {
                 if (inside_q) {
                     double sL = 0.0, sn = 0.0;
-                    for (int m1 = 2 + lane; m1 <= q; m1 += X::W) { sL += P.lap(m1); sn += P.count(m1); }
+                    for (int m1 = 2 + lane; m1 <= q; m1 += W) { sL += P.lap(m1); sn += P.count(m1); }
                     sL = X::sum(sL);
                     sn = X::sum(sn);
                     if (sn > 0.0) d2F = sL / sn;
@@ -636,7 +648,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     }
                     d2F = nav > 0 ? sacc / (double)nav : 0.0;
                 }
-                if constexpr (COOP) d2F = __shfl(d2F, 0, 64);
+                if constexpr (W > 1) d2F = __shfl(d2F, X::base(), 64);
             }
             if (X::leader()) {
                 for (int L = 0; L < 3; L++) { R.Sbar[o * 3 + L] = Sb[L]; R.Sbar2[o * 3 + L] = Sb2[L]; }
@@ -663,7 +675,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     int f = first_rad_gt(C, 1, m - 1, rcur);
                     m0_rc = f <= m - 1 ? f : m - 1;
                 }
-                Strain src = kernel_strain<COOP>(P, C, m0_rc, mlow_rc, mupp_rc);
+                Strain src = kernel_strain<W>(P, C, m0_rc, mlow_rc, mupp_rc);
                 if (!finite6(src.E)) { status = DUMP_NONFINITE; goto nocol; }
                 double Frho_rc, e_rc, p_rc;
                 evals_ep(src.E, Frho_rc, e_rc, p_rc);

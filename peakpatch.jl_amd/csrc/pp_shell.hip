@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(64) k_shell_plain(GatherFields F, const PeakPa
     __threadfence_block();
     __syncthreads();
     GlobalProfile P{pf, fb, C.rad, m, F.lap != nullptr};
-    analyse_post<true, true>(P, pk, C, R, pk.order);
+    analyse_post<64, true>(P, pk, C, R, pk.order);
     if (lane == 0) R.nshells[pk.order] = m;
 }
 
@@ -668,9 +668,11 @@ int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
     static const int cand_chunk[] = {32, 64, 128, 256};
     static const int cand_coop[] = {32, 64, 128, 256, 1 << 20};
     static const int cand_waves[] = {8, 12, 16, 24, 32};
+    static const int cand_width[] = {1, 4, 8, 16};
     struct Knob { int *v; const int *c; int nc; };
     const Knob knobs[] = {{&ctx->chunk_shells, cand_chunk, 4}, {&ctx->coop_m, cand_coop, 5},
-                          {&ctx->gather_waves_cu, cand_waves, 5}, {&ctx->tiny_m, cand_small, 4}};
+                          {&ctx->post_width, cand_width, 4}, {&ctx->gather_waves_cu, cand_waves, 5},
+                          {&ctx->tiny_m, cand_small, 4}};
     for (const Knob &k : knobs) {
         int bestv = *k.v;
         double bt = 1e300;
