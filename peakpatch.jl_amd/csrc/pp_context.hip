@@ -283,6 +283,7 @@ int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get)
         if (set[2] > 0) ctx->coop_m = set[2];
         if (set[3] > 0) ctx->gather_waves_cu = set[3];
         if (set[4] > 0) ctx->post_width = set[4];
+        if (set[5] >= 0) ctx->lane_walk = set[5];
     }
     if (get) {
         get[0] = ctx->tiny_m;
@@ -290,6 +291,7 @@ int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get)
         get[2] = ctx->coop_m;
         get[3] = ctx->gather_waves_cu;
         get[4] = ctx->post_width;
+        get[5] = ctx->lane_walk;
     }
     return PP_OK;
 }

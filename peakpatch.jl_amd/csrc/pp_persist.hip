@@ -485,18 +485,40 @@ struct AosProfile {
 #define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
 #endif
 
+// One lane per peak.  A walk longer than max_steps (a wave runs in step with
+// its slowest lane) is handed to the wave post through the defer list.
 template <bool FULL>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST_LANE_WAVES))) k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
-                                                  float *__restrict__ prof, const int64_t *__restrict__ prof_off,
-                                                  PostConst C, ResultView R, int has_lap)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST_LANE_WAVES)))
+k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, float *__restrict__ prof,
+            const int64_t *__restrict__ prof_off, PostConst C, ResultView R, int has_lap, int max_steps,
+            int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += stride) {
         const PeakParam pk = pps[idx];
         float *rec = prof + prof_off[idx] * PR;
         AosProfile<FULL> P{rec, C.rad, has_lap != 0};
-        analyse_post<1, FULL>(P, pk, C, R, pk.order);
+        if (analyse_post<1, FULL>(P, pk, C, R, pk.order, max_steps) == POST_DEFER) {
+            defer[atomicAdd(ndefer, 1u)] = idx;
+            continue;
+        }
         R.nshells[pk.order] = pk.m;
+    }
+}
+
+// One wave per deferred peak.
+__global__ void __launch_bounds__(64) k_post_defer(const PeakParam *__restrict__ pps, const int64_t *__restrict__ defer,
+                                                   const unsigned int *__restrict__ ndefer, float *__restrict__ prof,
+                                                   const int64_t *__restrict__ prof_off, PostConst C, ResultView R)
+{
+    const unsigned int n = *ndefer;
+    for (unsigned int t = blockIdx.x; t < n; t += gridDim.x) {
+        const int64_t idx = defer[t];
+        const PeakParam pk = pps[idx];
+        float *rec = prof + prof_off[idx] * PR;
+        AosProfile<false> P{rec, C.rad, false};
+        analyse_post<64, false>(P, pk, C, R, pk.order);
+        if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
     }
 }
 
@@ -1024,15 +1046,29 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
                            d_rec, (const int64_t *)d_off, C, S.R);
         PP_HIP(hipGetLastError());
     }
+    // defer list of the lane post: [count | pad | indices]
+    PP_CHECK(ctx->d_k2.grow(256 + (size_t)std::max<int64_t>(small_lo - coop_end, 1) * sizeof(int64_t)));
+    unsigned int *d_ndefer = ctx->d_k2.as<unsigned int>();
+    int64_t *d_defer = (int64_t *)(ctx->d_k2.as<char>() + 256);
     auto post_lane = [&](int64_t lo, int64_t hi, bool full) -> int {
         if (hi <= lo) return PP_OK;
         const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64, (int64_t)ctx->num_cu * 16));
-        if (full)
+        if (full) {
             hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
-                               d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0);
-        else
+                               d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0, 0x7fffffff, d_defer,
+                               d_ndefer);
+        } else {
+            const int budget = ctx->lane_walk > 0 ? ctx->lane_walk : 0x7fffffff;
+            hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, (int *)d_ndefer, (int64_t)1, 0);
             hipLaunchKernelGGL(k_post_lane<false>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
-                               d_rec, (const int64_t *)d_off, C, S.R, 0);
+                               d_rec, (const int64_t *)d_off, C, S.R, 0, budget, d_defer, d_ndefer);
+            if (ctx->lane_walk > 0) {
+                const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(hi - lo, (int64_t)ctx->num_cu * 16));
+                hipLaunchKernelGGL(k_post_defer, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps,
+                                   (const int64_t *)d_defer, (const unsigned int *)d_ndefer, d_rec,
+                                   (const int64_t *)d_off, C, S.R);
+            }
+        }
         PP_HIP(hipGetLastError());
         return PP_OK;
     };

@@ -34,7 +34,8 @@ enum : int {
 };
 
 enum : int { DUMP_NONE = 0, DUMP_PREMASK = 1, DUMP_NO_FCRIT = 2, DUMP_M0_ONE = 3, DUMP_NO_COLLAPSE = 4,
-             DUMP_RTHL_NEG = 5, DUMP_NONFINITE = 6 };
+             DUMP_RTHL_NEG = 5, DUMP_NONFINITE = 6,
+             POST_DEFER = -1 /* not a status: the walk hit its step budget, nothing was written */ };
 
 struct PeakParam {
     int32_t ci, cj, ck;  // 0-based centre
@@ -420,8 +421,11 @@ __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, in
 // count(s) and strain_inputs(s); with FINISH it also provides S / S2 / lap
 // sums per shell and computes Sbar, Sbar2, d2F and the mask here; without
 // FINISH those are left to finish_cells() and the final m0 is recorded.
+// With a step budget, a walk longer than max_steps returns POST_DEFER
+// before anything is written.
 template <int W, bool FINISH, class PV>
-__device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C, const ResultView &R, int64_t o)
+__device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C, const ResultView &R, int64_t o,
+                            int max_steps = 0x7fffffff)
 {
     using X = Exec<W>;
     const int lane = X::lane();
@@ -505,6 +509,8 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         } else {
             if (m0 == 1) { status = DUMP_M0_ONE; goto nocol; }
             for (int mp = m0 - 1; mp >= 1; mp--) {
+                // a caller with a step budget re-runs long walks elsewhere
+                if (m0 - 1 - mp >= max_steps) return POST_DEFER;
                 double rupp_mp = P.rad(mp) + 2.0;
                 double rlow_mp = fmax(P.rad(mp) - 2.0, 0.0);
                 if (P.rad(mupp) > rupp_mp) {

@@ -115,17 +115,18 @@ class Context:
     def synchronize(self):
         L.check(self.lib.pp_ctx_synchronize(self.h))
 
-    TUNABLES = ("small_m", "chunk_shells", "coop_m", "gather_waves_cu", "post_width")
+    TUNABLES = ("small_m", "chunk_shells", "coop_m", "gather_waves_cu", "post_width", "lane_walk")
 
     def tunables(self, **kw):
         """Get (and with keywords, set) the persistent form's launch
-        parameters: small_m, chunk_shells, coop_m, gather_waves_cu, post_width."""
+        parameters: small_m, chunk_shells, coop_m, gather_waves_cu, post_width,
+        lane_walk."""
         unknown = set(kw) - set(self.TUNABLES)
         if unknown:
             raise ValueError(f"unknown tunables {sorted(unknown)}")
         get = np.zeros(len(self.TUNABLES), np.int32)
         if kw:
-            st = np.array([-1] + [0] * (len(self.TUNABLES) - 1), np.int32)
+            st = np.array([-1, 0, 0, 0, 0, -1], np.int32)
             for q, name in enumerate(self.TUNABLES):
                 if name in kw:
                     st[q] = int(kw[name])
