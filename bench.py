@@ -166,6 +166,7 @@ def main():
                     help="tiles in the serving stream (Poisson arrivals, nmesh in {128..768}); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="tile arrival rate, tiles/s")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving stream (a test of it)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_gather_c3_pmc.txt"),
                     help="rocprofv3 --pmc summary whose 'traffic_bytes_per_launch' fills roofline.traffic")
     args = ap.parse_args()
@@ -182,10 +183,14 @@ def main():
     cfg = CONFIGS[args.config]
     ctx = Context(0)
     ct, table_name = load_table(ctx)
+    if args.serve_only:
+        print(json.dumps({"serving": serve_stream(args, ct, ctx.tunables())}), flush=True)
+        return
     ctx.build_shell_tables(cfg.nhunt)
     t0 = time.perf_counter()
     tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1,
-                                 device="cuda")
+                                 device="cuda", ctx=ctx)
+    ctx.synchronize()
     torch.cuda.synchronize()
     log(f"[bench] {cfg.describe()}; fields synthesised in {time.perf_counter() - t0:.1f} s")
     drv = TileDriver(cfg)

@@ -218,6 +218,13 @@ int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get);
 int pp_last_timing(pp_ctx *ctx, double *shell_gather_ms, double *shell_total_ms,
                    double *peakfind_ms);
 
+/* ------------------------------------------------------ input synthesis */
+/* White noise of fill_noise_threefry! (src/InitialConditions/RandomField.jl:
+ * 269-306) into out (device, ncells Float32, x fastest), on the context
+ * stream: Threefry-2x64-20 counter pairs under key (seed, 0), Box-Muller in
+ * Float64.  The tile synthesis uses it; it is not on the analysis path. */
+int pp_fill_noise_threefry(pp_ctx *ctx, float *out, int64_t ncells, uint64_t seed);
+
 /* ------------------------------------------------------- halo records */
 /* Per-peak inputs of the record packing besides the pp_results: the host
  * bookkeeping of the drivers (Pipeline.jl:217-231, 257-271).  Device arrays. */

@@ -470,3 +470,56 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ input noise
+// fill_noise_threefry! (src/InitialConditions/RandomField.jl:269-306): cell
+// idx (1-based, x fastest) takes the first normal of counter pair
+// (idx-1) >> 1 when idx-1 is even, the second otherwise; each pair is
+// Threefry-2x64-20 of counter (pair, 0) under key (seed, 0) (Salmon et al.
+// 2011), two 53-bit uniforms in (0, 1] and Box-Muller in Float64.
+namespace pp {
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+__global__ void k_noise_threefry(float *__restrict__ out, int64_t n3, uint64_t seed)
+{
+    constexpr int ROT[8] = {16, 42, 12, 31, 16, 32, 24, 21};
+    const uint64_t ks[3] = {seed, 0ull, 0x1BD11BDAA9FC1A22ull ^ seed};
+    const int64_t npairs = (n3 + 1) / 2;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t x0 = (uint64_t)p + ks[0], x1 = ks[1];
+#pragma unroll
+        for (int blk = 0; blk < 5; blk++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                x0 += x1;
+                x1 = rotl64(x1, ROT[(4 * blk + r) & 7]);
+                x1 ^= x0;
+            }
+            const int inj = blk + 1;
+            x0 += ks[inj % 3];
+            x1 += ks[(inj + 1) % 3] + (uint64_t)inj;
+        }
+        const double u1 = ((double)(x0 >> 11) + 1.0) * 1.1102230246251565e-16;
+        const double u2 = ((double)(x1 >> 11) + 1.0) * 1.1102230246251565e-16;
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 6.283185307179586 * u2;
+        out[2 * p] = (float)(r * cos(th));
+        if (2 * p + 1 < n3) out[2 * p + 1] = (float)(r * sin(th));
+    }
+}
+
+}  // namespace pp
+
+extern "C" int pp_fill_noise_threefry(pp_ctx *ctx, float *out, int64_t ncells, uint64_t seed)
+{
+    PP_ARG(ctx && out, "NULL argument");
+    PP_ARG(ncells >= 0, "negative cell count");
+    PP_CHECK(pp::ensure_device(ctx));
+    if (ncells == 0) return PP_OK;
+    const int64_t npairs = (ncells + 1) / 2;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((npairs + 255) / 256, (int64_t)ctx->num_cu * 16));
+    hipLaunchKernelGGL(pp::k_noise_threefry, dim3(grid), dim3(256), 0, ctx->stream, out, ncells, seed);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}

@@ -85,6 +85,7 @@ SIGNATURES = {
     "pp_last_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]),
     "pp_shell_tunables": (_i, [_vp, _vp, _vp]),
     "pp_last_shell_stages": (_i, [_vp, C.POINTER(_d)]),
+    "pp_fill_noise_threefry": (_i, [_vp, _vp, _i64, C.c_uint64]),
     "pp_pack_halos": (_i, [_vp, C.POINTER(Results), C.POINTER(HaloInputs), _d, _d, _i, _vp, _i64, _i,
                            C.POINTER(_i64), C.POINTER(_f)]),
 }

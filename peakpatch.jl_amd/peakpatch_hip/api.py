@@ -112,6 +112,15 @@ class Context:
         L.check(self.lib.pp_ctx_stream(self.h, C.byref(s)))
         return int(s.value or 0)
 
+    def fill_noise(self, n: int, seed: int, out=None):
+        """Threefry white noise of an n^3 tile as a CUDA float32 tensor
+        (RandomField.jl:269-306), generated on the context's stream."""
+        import torch
+        if out is None:
+            out = torch.empty((n, n, n), dtype=torch.float32, device=f"cuda:{self.device}")
+        L.check(self.lib.pp_fill_noise_threefry(self.h, out.data_ptr(), n * n * n, int(seed) & 0xFFFFFFFFFFFFFFFF))
+        return out
+
     def synchronize(self):
         L.check(self.lib.pp_ctx_synchronize(self.h))
 

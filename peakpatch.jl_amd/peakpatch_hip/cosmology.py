@@ -127,13 +127,16 @@ class GrowthTables:
         return 1.686 * self.D(1.0 / (1.0 + z)).reshape(z.shape)
 
 
+_GL64 = np.polynomial.legendre.leggauss(64)
+
+
 def chi(z: float, c: CosmologyParams) -> float:
     """chi(z) (Cosmology.jl:21-24): (2.998e5/100) h int_0^z dz/E(z) [the
     reference's h factor kept]; 64-point Gauss-Legendre per unit redshift."""
     if z <= 0.0:
         return 0.0
     nseg = max(1, int(math.ceil(z)))
-    x, w = np.polynomial.legendre.leggauss(64)
+    x, w = _GL64
     tot = 0.0
     for s in range(nseg):
         a, b = z * s / nseg, z * (s + 1) / nseg

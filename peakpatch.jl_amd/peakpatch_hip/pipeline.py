@@ -124,13 +124,27 @@ class PeakBatch:
                 "D": t(self.D, torch.float64), "v2fac": t(self.v2fac, torch.float64)}
 
 
+_TABLES: dict = {}
+
+
+def _tables(cosmology, z_max: float, ievol: int):
+    """Growth and chi->z tables depend on the cosmology only (built once per
+    process, as the drivers build them once per run)."""
+    key = (cosmology, z_max if ievol == 1 else None)
+    t = _TABLES.get(key)
+    if t is None:
+        t = (cosmo_mod.GrowthTables(cosmology),
+             cosmo_mod.ChiToZ(cosmology, z_max=z_max + 1.0) if ievol == 1 else None)
+        _TABLES[key] = t
+    return t
+
+
 class TileDriver:
     """Holds the per-process host state a tile needs (growth tables, chi->z)."""
 
     def __init__(self, cfg: TileConfig):
         self.cfg = cfg
-        self.growth = cosmo_mod.GrowthTables(cfg.cosmology)
-        self.chi2z = cosmo_mod.ChiToZ(cfg.cosmology, z_max=cfg.z_max + 1.0) if cfg.ievol == 1 else None
+        self.growth, self.chi2z = _tables(cfg.cosmology, cfg.z_max, cfg.ievol)
 
     def filter_fcrit(self) -> np.ndarray:
         """fcrit per filter: fsc_of_z(z_out), or at the tile centre in light-cone

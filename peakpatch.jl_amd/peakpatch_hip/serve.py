@@ -28,8 +28,6 @@ and failure handling are testable without a GPU.
 from __future__ import annotations
 
 import collections
-import dataclasses
-import math
 import threading
 import time
 from concurrent.futures import Future
@@ -69,6 +67,7 @@ class TileResult:
     done: float = 0.0
     slot: int = -1
     error: str | None = None
+    stages: dict = field(default_factory=dict)  # seconds per pipeline stage (host clock)
 
     @property
     def latency(self) -> float:
@@ -239,7 +238,12 @@ def summarize(results) -> dict:
     ok = [r for r in results if r.error is None]
     lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
     wall = (max(r.done for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
+    stages = {}
+    for r in ok:
+        for k, v in r.stages.items():
+            stages[k] = stages.get(k, 0.0) + v
     return {"tiles": len(results), "completed": len(ok), "failed": len(results) - len(ok),
+            "mean_stage_s": {k: v / len(ok) for k, v in stages.items()} if ok else {},
             "tiles_per_s": len(ok) / wall if wall > 0 else 0.0,
             "p50_ms": float(np.percentile(lat, 50)) * 1e3 if len(lat) else None,
             "p99_ms": float(np.percentile(lat, 99)) * 1e3 if len(lat) else None,
@@ -283,12 +287,24 @@ class GpuTileRunner:
         ctx, stream = state["ctx"], state["stream"]
         cfg = tile_config(req, self.template)
         drv = TileDriver(cfg)
+        st = {}
+        t = [time.perf_counter()]
+
+        def lap(name):
+            stream.synchronize()
+            now = time.perf_counter()
+            st[name] = now - t[0]
+            t[0] = now
+
         with torch.cuda.stream(stream):
             tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt,
-                                         want_lapd=cfg.ioutshear >= 1, device=f"cuda:{self.device}")
+                                         want_lapd=cfg.ioutshear >= 1, device=f"cuda:{self.device}", ctx=ctx)
+            lap("synthesis")
             peaks = drv.find_peaks(ctx, tile.delta)
+            lap("peak_find")
             b = drv.batch(peaks)
-            out = TileResult(req.tile_id, req.nmesh, npeaks=len(b))
+            lap("bookkeeping")
+            out = TileResult(req.tile_id, req.nmesh, npeaks=len(b), stages=st)
             if len(b) == 0:
                 out.records = np.zeros((0, 33), np.float32)
                 return out
@@ -296,9 +312,10 @@ class GpuTileRunner:
             mask = torch.zeros((cfg.n,) * 3, dtype=torch.int8, device=f"cuda:{self.device}")
             fields = (tile.delta, tile.eta, tile.eta2, tile.lapd)
             res = drv.analyse(ctx, fields, bdev, mask=mask, kernel=SHELL_PERSISTENT, device_out=True)
+            lap("shell_analysis")
             rec, rmax = catalog.pack_halos(ctx, res, bdev, cfg.alatt, extended=True, z_max=cfg.z_max)
             out.records = rec.cpu().numpy()
-            stream.synchronize()
+            lap("records")
         out.RTHLmax = rmax
         out.nhalo = len(out.records)
         return out

@@ -306,12 +306,13 @@ def _kgrid_torch(n, boxsize, device):
 
 
 def make_tile_torch(n: int, boxsize: float, seed: int = 42, ilpt: int = 2, want_lapd: bool = True,
-                    pk=None, device="cuda"):
+                    pk=None, device="cuda", ctx=None):
     """make_tile on a torch device; returns TileInputs holding torch tensors
-    (delta_k stays on the device as complex64)."""
+    (delta_k stays on the device as complex64).  With a Context the white
+    noise comes from the library's Threefry kernel on its stream."""
     torch = _torch()
     pk = pk or EH98NoWiggle()
-    noise = fill_noise_threefry_torch(n, seed, device)
+    noise = ctx.fill_noise(n, seed) if ctx is not None else fill_noise_threefry_torch(n, seed, device)
     kx, ky, kz = _kgrid_torch(n, boxsize, device)
     k2 = kx * kx + ky * ky + kz * kz
     dk = 2.0 * math.pi / boxsize
