@@ -10,10 +10,12 @@ GPU, the filter bank + stencil finds the peaks, the host computes per-peak
 redshift / fcrit.  A step is one shell analysis of every peak of the tile
 (gather + virialisation walk + R_TH + mask), fields resident in HBM.
 
-Prints one JSON line: shell-analysed peaks/s, the roofline of the shell
-kernel (algorithmic bytes = 4 B x F fields x cells gathered), and the CPU
-baseline (the FP64 restatement of the reference algorithm, OpenMP over peaks)
-timed on a bounded stratified sample of the same peaks.
+Prints one JSON line: shell-analysed peaks/s, the roofline of the dominant
+kernel (the wave gather: 4 B x 5 fields x the cells it reads, against its
+HIP-event time; the whole call's 4 B x F fields x cells figure is reported
+beside it), the p50/p99 latency of a stream of tiles through the serving
+loop, and the CPU baseline (the FP64 restatement of the reference algorithm,
+OpenMP over peaks) timed on a bounded stratified sample of the same peaks.
 """
 from __future__ import annotations
 
@@ -162,9 +164,9 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="before the timed steps, run one elementwise kernel over a GIB-sized float32 buffer "
                          "(known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--serve-tiles", type=int, default=16,
+    ap.add_argument("--serve-tiles", type=int, default=12,
                     help="tiles in the serving stream (Poisson arrivals, nmesh in {128..768}); 0 skips it")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="tile arrival rate, tiles/s")
+    ap.add_argument("--serve-rate", type=float, default=0.4, help="tile arrival rate, tiles/s")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream (a test of it)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_gather_c3_pmc.txt"),
@@ -255,9 +257,11 @@ def main():
     stage_ms = {k: float(np.mean([st[k] for st in stages])) for k in stages[0]}
     if head == "persistent":
         # dominant kernel: the wave gather (k_gather) over the peaks above the
-        # small-peak threshold; bytes = 4 B x F fields x cells it reads
+        # small-peak threshold; it reads 5 fields per gathered cell (delta,
+        # its transpose, eta x/y/z; eta2 and lapd are read by the finish pass
+        # inside R_TH only), so its bytes are 4 B x 5 x the cells it reads
         big = mshell > tuned["small_m"]
-        dom_bytes = float(4 * F * cells[big].sum())
+        dom_bytes = float(4 * 5 * cells[big].sum())
         dom_s = stage_ms["gather"] / 1e3
         dom_name = "k_gather"
     else:
