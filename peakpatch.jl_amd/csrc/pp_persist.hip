@@ -48,6 +48,14 @@ enum : int {
     P_FB = 14 /* Fbar: double in slots 14-15 */, P_S = 16, P_S2 = 19, P_L = 22
 };
 constexpr int SMALL_MAX = 16;  // largest m whose groups fit one 64-lane batch (62 groups)
+// Work placement.  The gathers read the fields around each peak; peaks of
+// equal m are taken in Morton order of their centres so that the waves in
+// flight share cache lines, and each XCD (its own L2) works through its own
+// share of every m: workgroup b runs on XCD b % NXCD (round-robin dispatch).
+constexpr int NXCD = 8;
+#ifndef PP_LOCALITY
+#define PP_LOCALITY 2  // 0: one list in m order, 1: per-XCD lists, 2: + Morton order in the lists
+#endif
 
 // ------------------------------------------------------------- transpose
 __global__ void __launch_bounds__(256) k_transpose(const float *__restrict__ in, float *__restrict__ out, int n)
@@ -114,6 +122,48 @@ __global__ void k_m_to_i64(const int32_t *__restrict__ m, int64_t n, int64_t *__
 __global__ void k_fill_i32b(int *p, int64_t n, int v)
 {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+// bits 0..20 of v moved to every third bit (Morton interleave)
+__device__ __forceinline__ uint64_t spread3(uint32_t v)
+{
+    uint64_t x = v & 0x1fffffu;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+// sort key: m descending, then the Morton code of the centre cell (nbits per
+// axis), so peaks of equal m that are processed together lie close in space
+__global__ void k_keys_morton(const PeakParam *__restrict__ pp, int64_t np, int nbits, uint64_t *__restrict__ keys)
+{
+    const uint32_t msk = (1u << nbits) - 1u;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const PeakParam q = pp[p];
+        const uint64_t mort = spread3((uint32_t)q.ci & msk) | spread3((uint32_t)q.cj & msk) << 1 |
+                              spread3((uint32_t)q.ck & msk) << 2;
+        keys[p] = (uint64_t)(0xffff - min(q.m, 0xffff)) << (3 * nbits) | mort;
+    }
+}
+
+// Per-XCD work lists of sorted positions [0, hi): every run of equal m
+// (tab row m: first position, length, then the list offset of each of its
+// nx parts) is cut into nx parts of consecutive entries of src (the
+// positions themselves if src is null); part x is appended to list x.
+__global__ void k_xcd_order(const int32_t *__restrict__ m_sorted, const int32_t *__restrict__ src, int64_t hi,
+                            const int64_t *__restrict__ tab, int nx, int32_t *__restrict__ xorder)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < hi; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t *t = tab + (int64_t)m_sorted[p] * (2 + NXCD);
+        const int64_t r = p - t[0], len = t[1];
+        int x = (int)min<int64_t>(nx - 1, r * nx / len);
+        while (x + 1 < nx && len * (x + 1) / nx <= r) x++;
+        while (x > 0 && len * x / nx > r) x--;
+        xorder[t[2 + x] + r - len * x / nx] = src ? src[p] : (int32_t)p;
+    }
 }
 
 // ----------------------------------------------------------- group sums
@@ -208,20 +258,29 @@ __device__ __forceinline__ double wave_scan(double x)
 // in row-major order, so each load instruction covers whole x-runs of the
 // peaks' rows.  A block handles P = floor(64 / m) peaks of equal m; the host
 // cut their P x ncells cells into dense 64-lane batches and sorted every
-// batch by (peak, shell).  Each lane stages its cell's 21 contributions in LDS and the
-// first lane of each (peak, shell) run adds the run in lane order to that
-// shell's LDS bin.  Records carry S, S2 and lapd so the lane post can finish
-// these peaks itself.
+// batch by (peak, shell).  The lanes of each (peak, shell) run sum their
+// cells' 21 contributions by a segmented tree over wave shuffles (step d:
+// lane += lane + d while inside the run), and the run's first lane adds the
+// sum to that shell's LDS bin.  Records carry S, S2 and lapd so the lane
+// post can finish these peaks itself.
 struct CellSlot {
     int32_t cell;   // (di+128) | (dj+128) << 8 | (dk+128) << 16; -1 for an empty lane
     int16_t q, s;   // peak slot in the block, shell (1-based)
     int32_t run;    // length of the (q, s) run starting here, 0 if not its first lane
-    int32_t pad;
+    int32_t rend;   // lane one past the end of this lane's run
 };
 
 struct SmallSeg {
     int32_t m, P, nbat, toff;  // shells, peaks per block, batches per block, first CellSlot
     int64_t lo, hi, blk0;      // sorted peaks [lo, hi); first block
+    int32_t steps, pad;        // tree steps: 2^steps >= the longest run of the tables
+};
+
+// blocks [blk_lo, blk_lo + count) of segment seg, at position cum of an XCD's
+// block list (count = the next part's cum - cum)
+struct SmallPart {
+    int32_t seg, pad;
+    int64_t blk_lo, cum;
 };
 
 struct SmallArgs {
@@ -229,8 +288,10 @@ struct SmallArgs {
     const PeakParam *pps;
     const CellSlot *tab;
     const SmallSeg *seg;
-    int nseg;
-    int64_t nblk;
+    const SmallPart *part;     // list x: parts [xp[x], xp[x + 1]), xlen[x] blocks
+    int32_t xp[NXCD + 1];
+    int64_t xlen[NXCD];
+    int nx;                    // lists (1 or NXCD); gridDim.x is a multiple of nx
     float *prof;
     const int64_t *prof_off;
     const double *rad;
@@ -241,18 +302,20 @@ __device__ __forceinline__ constexpr int sf_slot(int f) { return f < P_NB ? f : 
 
 __global__ void __launch_bounds__(64) k_small(SmallArgs A)
 {
-    __shared__ float stg[NSF][64];
     __shared__ float bins[NSF][64];  // (peak q, shell s) at q m + s - 1
     __shared__ double sdF[64], sfb[64];
     const int lane = threadIdx.x;
     const GatherFields &F = A.F;
     const long n1 = F.n, n12 = (long)F.n * F.n;
-    for (int64_t blk = blockIdx.x; blk < A.nblk; blk += gridDim.x) {
-        int si = 0;
-        while (si + 1 < A.nseg && blk >= A.seg[si + 1].blk0) si++;
-        const SmallSeg S = A.seg[si];
+    const int x = blockIdx.x % A.nx;
+    const int64_t nloc = A.xlen[x], stride = gridDim.x / A.nx;
+    for (int64_t j = blockIdx.x / A.nx; j < nloc; j += stride) {
+        int e = A.xp[x];
+        while (e + 1 < A.xp[x + 1] && j >= A.part[e + 1].cum) e++;
+        const SmallPart pt = A.part[e];
+        const SmallSeg S = A.seg[pt.seg];
         const int m = S.m, P = S.P;
-        const int64_t pbase = S.lo + (blk - S.blk0) * P;
+        const int64_t pbase = S.lo + (pt.blk_lo + (j - pt.cum)) * P;
         for (int f = 0; f < NSF; f++) bins[f][lane] = 0.0f;
         __syncthreads();
         for (int b = 0; b < S.nbat; b++) {
@@ -285,17 +348,19 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
                     if (F.lap) acc[P_NB + 6] = F.lap[a];
                 }
             }
+            for (int k = 0; k < S.steps; k++) {
+                const int d = 1 << k;
+                const bool in = lane + d < t.rend;
 #pragma unroll
-            for (int f = 0; f < NSF; f++) stg[f][lane] = acc[f];
-            __syncthreads();
+                for (int f = 0; f < NSF; f++) {
+                    const float u = __shfl_down(acc[f], d, 64);
+                    if (in) acc[f] += u;
+                }
+            }
             if (live && t.run > 0) {
                 const int o = t.q * m + t.s - 1;
 #pragma unroll
-                for (int f = 0; f < NSF; f++) {
-                    float v = stg[f][lane];
-                    for (int r = 1; r < t.run; r++) v += stg[f][lane + r];
-                    bins[f][o] += v;
-                }
+                for (int f = 0; f < NSF; f++) bins[f][o] += acc[f];
             }
             __syncthreads();
         }
@@ -338,7 +403,6 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
 struct GatherArgs {
     GatherFields F;
     const PeakParam *pps;
-    int64_t lo, hi;
     const GroupDesc *desc;     // all batch tables
     const int64_t *full_off;   // per chunk c: first descriptor of the full-chunk table
     const int32_t *full_nb;    //   and its number of 64-slot batches
@@ -347,7 +411,10 @@ struct GatherArgs {
     float *prof;
     const int64_t *prof_off;   // per sorted index: first shell record
     const double *rad;
-    int *queue;
+    int *queue;                // nx counters, 64 ints apart
+    const int32_t *xorder;     // per-XCD lists of sorted positions
+    int64_t xb[NXCD + 1];      // list x = xorder[xb[x], xb[x + 1])
+    int nx;                    // lists (1 or NXCD)
     int cs;                    // shells per chunk
 };
 
@@ -373,12 +440,20 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
     const int cs = A.cs;
     float *stg = lds;                  // [P_NB][64]
     float *bh = lds + P_NB * 64, *bl = bh + P_NB * cs;
+    // own XCD's list first, then the others' (the tail of the work)
+    const int x0 = blockIdx.x % A.nx;
+    int xq = 0;
     for (;;) {
-        int it = 0;
-        if (lane == 0) it = atomicAdd(A.queue, 1);
-        it = __shfl(it, 0, 64);
-        const int64_t idx = A.lo + it;
-        if (idx >= A.hi) break;
+        int idx = -1;
+        if (lane == 0) {
+            for (; xq < A.nx; xq++) {
+                const int x = (x0 + xq) % A.nx;
+                const int64_t t = A.xb[x] + atomicAdd(A.queue + 64 * x, 1);
+                if (t < A.xb[x + 1]) { idx = A.xorder[t]; break; }
+            }
+        }
+        idx = __shfl(idx, 0, 64);
+        if (idx < 0) break;
         const PeakParam pk = A.pps[idx];
         const int m = pk.m;
         float *rec = A.prof + A.prof_off[idx] * PR;
@@ -566,12 +641,14 @@ __global__ void __launch_bounds__(64) k_post_group(const PeakParam *__restrict__
 struct FinishArgs {
     GatherFields F;
     const PeakParam *pps;
-    int64_t lo, hi;
     const GroupDesc *desc;
     const int64_t *full_off;
     const int32_t *full_nb;
     const int64_t *part_off;
     const int32_t *part_nb;
+    const int32_t *xorder;     // per-XCD lists of sorted positions, as for k_gather
+    int64_t xb[NXCD + 1];
+    int nx;                    // gridDim.x is a multiple of nx
     int cs;
     PostConst C;
     ResultView R;
@@ -587,8 +664,10 @@ __global__ void __launch_bounds__(64) k_finish(FinishArgs A)
     const float *e2[3] = {F.e2x, F.e2y, F.e2z};
     const long n1 = F.n, n12 = (long)F.n * F.n;
     const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff;
-    for (int64_t idx = A.lo + blockIdx.x; idx < A.hi; idx += gridDim.x) {
-        const PeakParam pk = A.pps[idx];
+    const int x = blockIdx.x % A.nx;
+    const int64_t stride = gridDim.x / A.nx;
+    for (int64_t j = A.xb[x] + blockIdx.x / A.nx; j < A.xb[x + 1]; j += stride) {
+        const PeakParam pk = A.pps[A.xorder[j]];
         const int64_t o = pk.order;
         if (R.status[o] != DUMP_NONE) continue;
         const int m = pk.m, m0 = R.m0[o], q = R.q[o];
@@ -670,7 +749,7 @@ __global__ void __launch_bounds__(64) k_finish(FinishArgs A)
 
 // ------------------------------------------------------------- host side
 struct SmallTab {
-    int32_t toff = 0, P = 0, nbat = 0;
+    int32_t toff = 0, P = 0, nbat = 0, steps = 0;
 };
 
 struct PersistState {
@@ -683,7 +762,9 @@ struct PersistState {
     std::vector<SmallTab> stab;                       // index m (0..small_max)
     int small_max = 0;
     std::vector<SmallSeg> segs_host;
-    DevBuf d_desc, d_full_off, d_full_nb, d_part_off, d_part_nb, d_scells, d_seg;
+    std::vector<SmallPart> parts_host;                // per-XCD block lists of k_small
+    std::vector<int64_t> xtab_host;                   // per m: k_xcd_order's table row
+    DevBuf d_desc, d_full_off, d_full_nb, d_part_off, d_part_nb, d_scells, d_seg, d_xtab;
     bool dirty = true;
 };
 
@@ -703,7 +784,7 @@ void persist_release(pp_ctx *ctx)
     if (it == g_state.end()) return;
     PersistState &st = it->second;
     DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb, &st.d_scells,
-                      &st.d_seg};
+                      &st.d_seg, &st.d_xtab};
     for (DevBuf *b : bufs) b->release();
     g_state.erase(it);
 }
@@ -773,7 +854,7 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
     const ShellTable &T = ctx->shells;
     if (st.rmax != T.rmax || st.cs != cs) {
         DevBuf *bufs[] = {&st.d_desc, &st.d_full_off, &st.d_full_nb, &st.d_part_off, &st.d_part_nb, &st.d_scells,
-                          &st.d_seg};
+                          &st.d_seg, &st.d_xtab};
         for (DevBuf *b : bufs) b->release();
         st = PersistState();
         st.rmax = T.rmax;
@@ -812,7 +893,7 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
                     c.q = (int16_t)(v / nc);
                     c.s = cells[v % nc].second;
                     c.run = 0;
-                    c.pad = 0;
+                    c.rend = 0;
                     bt.push_back(c);
                 }
                 std::stable_sort(bt.begin(), bt.end(), [](const CellSlot &x, const CellSlot &y) {
@@ -822,9 +903,11 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
                     size_t r = q0;
                     while (r < bt.size() && bt[r].q == bt[q0].q && bt[r].s == bt[q0].s) r++;
                     bt[q0].run = (int32_t)(r - q0);
+                    for (size_t u = q0; u < r; u++) bt[u].rend = (int32_t)r;
+                    while ((1 << t.steps) < (int)(r - q0)) t.steps++;
                     q0 = r;
                 }
-                while (bt.size() < 64) bt.push_back(CellSlot{-1, 0, 1, 0, 0});
+                while (bt.size() < 64) bt.push_back(CellSlot{-1, 0, 1, 0, (int32_t)bt.size() + 1});
                 st.scells.insert(st.scells.end(), bt.begin(), bt.end());
             }
             st.stab[m] = t;
@@ -882,18 +965,33 @@ static int grid1d(pp_ctx *ctx, int64_t n, int per_cu = 8)
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cu * per_cu));
 }
 
-// Ordered params + histogram of m; returns host histogram.
-static int order_peaks(pp_ctx *ctx, ShellRun &S, PeakParam *&pps, int32_t *&m_sorted, std::vector<int> &hist)
+// Ordered params + histogram of m; returns host histogram.  The order is m
+// descending, ties in the caller's order (so the lane posts' result writes
+// stay coalesced).  W: scratch for the work lists.
+struct ListScratch {
+    int32_t *xg, *xf;   // per-XCD lists: gather, finish
+    int32_t *iota, *mo; // Morton sort values
+    uint64_t *ka, *kb;  // Morton sort keys
+};
+
+static int order_peaks(pp_ctx *ctx, ShellRun &S, PeakParam *&pps, int32_t *&m_sorted, ListScratch &W,
+                       std::vector<int> &hist)
 {
     const int64_t np = S.np;
     const int nb = ctx->shells.nshells + 1;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    PP_CHECK(ctx->d_queue.grow(al(np * sizeof(PeakParam)) + al(np * 4) * 3 + al(nb * 4) + 4096));
+    PP_CHECK(ctx->d_queue.grow(al(np * sizeof(PeakParam)) + al(np * 4) * 7 + al(np * 8) * 2 + al(nb * 4) + 4096));
     char *p = ctx->d_queue.as<char>();
     pps = (PeakParam *)p; p += al(np * sizeof(PeakParam));
     m_sorted = (int32_t *)p; p += al(np * 4);
     int32_t *order2 = (int32_t *)p; p += al(np * 4);
     int32_t *keys2 = (int32_t *)p; p += al(np * 4);
+    W.xg = (int32_t *)p; p += al(np * 4);
+    W.xf = (int32_t *)p; p += al(np * 4);
+    W.iota = (int32_t *)p; p += al(np * 4);
+    W.mo = (int32_t *)p; p += al(np * 4);
+    W.ka = (uint64_t *)p; p += al(np * 8);
+    W.kb = (uint64_t *)p; p += al(np * 8);
     int *d_hist = (int *)p; p += al(nb * 4);
     int *d_flag = (int *)p;
     hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, d_flag, (int64_t)1, 0);
@@ -925,6 +1023,25 @@ static int order_peaks(pp_ctx *ctx, ShellRun &S, PeakParam *&pps, int32_t *&m_so
     return PP_OK;
 }
 
+// Positions [0, hi) of the ordered params with ties of m in Morton order of
+// the centres in the n^3 grid (W.mo).
+static int morton_order(pp_ctx *ctx, const PeakParam *pps, int64_t hi, int n, ListScratch &W)
+{
+    int nbits = 1;
+    while ((1 << nbits) < n) nbits++;
+    if (nbits > 16) { set_error("grid side %d above 65536", n); return PP_ERR_ARG; }
+    hipLaunchKernelGGL(k_keys_morton, dim3(grid1d(ctx, hi)), dim3(256), 0, ctx->stream, pps, hi, nbits, W.ka);
+    hipLaunchKernelGGL(k_iota, dim3(grid1d(ctx, hi)), dim3(256), 0, ctx->stream, W.iota, hi);
+    PP_HIP(hipGetLastError());
+    size_t tmp = 0;
+    PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, W.ka, W.kb, W.iota, W.mo, (int)hi, 0, 3 * nbits + 16,
+                                              ctx->stream));
+    PP_CHECK(ctx->d_cub.grow(tmp));
+    PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, W.ka, W.kb, W.iota, W.mo, (int)hi, 0, 3 * nbits + 16,
+                                              ctx->stream));
+    return PP_OK;
+}
+
 // The persistent pipeline on an ordered batch.  export_all keeps the profile
 // for pp_shell_profiles (every peak then goes through k_gather).
 static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, bool export_all,
@@ -946,8 +1063,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // 2. order by m (descending)
     PeakParam *pps = nullptr;
     int32_t *m_sorted = nullptr;
+    ListScratch W;
     std::vector<int> hist;
-    PP_CHECK(order_peaks(ctx, S, pps, m_sorted, hist));
+    PP_CHECK(order_peaks(ctx, S, pps, m_sorted, W, hist));
     const int nb = (int)hist.size();
     std::vector<int64_t> above(nb + 1, 0);  // above[x] = #peaks with m >= x
     for (int x = nb - 1; x >= 0; x--) above[x] = above[x + 1] + hist[x];
@@ -985,7 +1103,55 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     }
     PP_CHECK(ctx->d_res.grow((size_t)std::max<int64_t>(tot, 1) * PR * sizeof(float) + 4096));
     float *d_rec = ctx->d_res.as<float>();
-    int *d_q = (int *)(ctx->d_misc.as<char>() + 1024);
+    PP_CHECK(ctx->d_misc.grow(8192));
+    int *d_q = (int *)(ctx->d_misc.as<char>() + 4096);  // NXCD queue counters, 256 B apart
+    // per-XCD lists of the wave-gathered peaks [0, small_lo): part x of every
+    // run of equal m goes to list x
+    const int nx = PP_LOCALITY >= 1 ? NXCD : 1;
+    int64_t xb[NXCD + 1];
+    {
+        st.xtab_host.assign((size_t)nb * (2 + NXCD), 0);
+        int64_t cur[NXCD] = {0};
+        for (int m = nb - 1; m > small_m; m--)
+            for (int x = 0; x < nx; x++) cur[x] += (int64_t)hist[m] * (x + 1) / nx - (int64_t)hist[m] * x / nx;
+        xb[0] = 0;
+        for (int x = 0; x < NXCD; x++) xb[x + 1] = xb[x] + (x < nx ? cur[x] : 0);
+        for (int x = 0; x < nx; x++) cur[x] = xb[x];
+        for (int m = nb - 1; m > small_m; m--) {
+            const int64_t len = hist[m];
+            if (!len) continue;
+            int64_t *t = &st.xtab_host[(size_t)m * (2 + NXCD)];
+            t[0] = cnt_gt(m);
+            t[1] = len;
+            for (int x = 0; x < nx; x++) {
+                t[2 + x] = cur[x];
+                cur[x] += len * (x + 1) / nx - len * x / nx;
+            }
+        }
+        if (xb[NXCD] != small_lo) {
+            set_error("per-XCD lists hold %lld of %lld peaks", (long long)xb[NXCD], (long long)small_lo);
+            return PP_ERR_STATE;
+        }
+        if (small_lo > 0) {
+            PP_CHECK(st.d_xtab.grow(st.xtab_host.size() * sizeof(int64_t)));
+            PP_HIP(hipMemcpyAsync(st.d_xtab.ptr, st.xtab_host.data(), st.xtab_host.size() * sizeof(int64_t),
+                                  hipMemcpyHostToDevice, ctx->stream));
+            // finish: parts of consecutive positions; gather: of the Morton order
+            hipLaunchKernelGGL(k_xcd_order, dim3(grid1d(ctx, small_lo)), dim3(256), 0, ctx->stream,
+                               (const int32_t *)m_sorted, (const int32_t *)nullptr, small_lo,
+                               st.d_xtab.as<int64_t>(), nx, W.xf);
+            PP_HIP(hipGetLastError());
+            if (PP_LOCALITY >= 2) {
+                PP_CHECK(morton_order(ctx, pps, small_lo, n, W));
+                hipLaunchKernelGGL(k_xcd_order, dim3(grid1d(ctx, small_lo)), dim3(256), 0, ctx->stream,
+                                   (const int32_t *)m_sorted, (const int32_t *)W.mo, small_lo,
+                                   st.d_xtab.as<int64_t>(), nx, W.xg);
+                PP_HIP(hipGetLastError());
+            } else {
+                W.xg = W.xf;
+            }
+        }
+    }
     // small-peak segments (equal m, descending)
     st.segs_host.clear();
     int64_t nblk = 0;
@@ -997,16 +1163,38 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         sg.P = st.stab[m].P;
         sg.nbat = st.stab[m].nbat;
         sg.toff = st.stab[m].toff;
+        sg.steps = st.stab[m].steps;
         sg.lo = lo;
         sg.hi = hi;
         sg.blk0 = nblk;
         nblk += (hi - lo + sg.P - 1) / sg.P;
         st.segs_host.push_back(sg);
     }
+    // per-XCD block lists: part x of every segment's blocks goes to list x
+    int32_t sxp[NXCD + 1] = {0};
+    int64_t sxlen[NXCD] = {0};
+    st.parts_host.clear();
+    for (int x = 0; x < nx; x++) {
+        sxp[x] = (int32_t)st.parts_host.size();
+        int64_t cum = 0;
+        for (size_t si = 0; si < st.segs_host.size(); si++) {
+            const SmallSeg &sg = st.segs_host[si];
+            const int64_t nbk = (sg.hi - sg.lo + sg.P - 1) / sg.P;
+            const int64_t b0 = nbk * x / nx, b1 = nbk * (x + 1) / nx;
+            if (b1 <= b0) continue;
+            st.parts_host.push_back(SmallPart{(int32_t)si, 0, b0, cum});
+            cum += b1 - b0;
+        }
+        sxlen[x] = cum;
+    }
+    for (int x = nx; x <= NXCD; x++) sxp[x] = (int32_t)st.parts_host.size();
+    const size_t seg_bytes = (st.segs_host.size() * sizeof(SmallSeg) + 255) & ~(size_t)255;
     if (!st.segs_host.empty()) {
-        PP_CHECK(st.d_seg.grow(st.segs_host.size() * sizeof(SmallSeg)));
+        PP_CHECK(st.d_seg.grow(seg_bytes + st.parts_host.size() * sizeof(SmallPart)));
         PP_HIP(hipMemcpyAsync(st.d_seg.ptr, st.segs_host.data(), st.segs_host.size() * sizeof(SmallSeg),
                               hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(st.d_seg.as<char>() + seg_bytes, st.parts_host.data(),
+                              st.parts_host.size() * sizeof(SmallPart), hipMemcpyHostToDevice, ctx->stream));
     }
     PP_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     // 4. gathers
@@ -1016,13 +1204,15 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         const int waves_cu = std::max(1, std::min<int>(ctx->gather_waves_cu > 0 ? ctx->gather_waves_cu : 16,
                                                        (int)(lds_max / lds)));
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * waves_cu));
-        hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, d_q, (int64_t)1, 0);
+        hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(256), 0, ctx->stream, d_q, (int64_t)(64 * NXCD), 0);
         GatherArgs A;
-        A.F = G; A.pps = pps; A.lo = 0; A.hi = small_lo;
+        A.F = G; A.pps = pps;
         A.desc = st.d_desc.as<GroupDesc>();
         A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
         A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
         A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs;
+        A.xorder = W.xg; A.nx = nx;
+        for (int x = 0; x <= NXCD; x++) A.xb[x] = xb[x];
         if (lds > 64 * 1024)
             (void)hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
@@ -1032,9 +1222,14 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     if (nblk > 0) {
         SmallArgs A;
         A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
-        A.seg = st.d_seg.as<SmallSeg>(); A.nseg = (int)st.segs_host.size(); A.nblk = nblk;
+        A.seg = st.d_seg.as<SmallSeg>();
+        A.part = (const SmallPart *)(st.d_seg.as<char>() + seg_bytes);
+        for (int x = 0; x <= NXCD; x++) A.xp[x] = sxp[x];
+        for (int x = 0; x < NXCD; x++) A.xlen[x] = sxlen[x];
+        A.nx = nx;
         A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
-        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
+        int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
+        grid = (grid + nx - 1) / nx * nx;
         hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->stream, A);
         PP_HIP(hipGetLastError());
     }
@@ -1092,9 +1287,12 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     // 6. finish for the wave-gathered peaks
     if (small_lo > 0) {
-        const int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * 16));
+        int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * 16));
+        gf = (gf + nx - 1) / nx * nx;
         FinishArgs A;
-        A.F = G; A.pps = pps; A.lo = 0; A.hi = small_lo;
+        A.F = G; A.pps = pps;
+        A.xorder = W.xf; A.nx = nx;
+        for (int x = 0; x <= NXCD; x++) A.xb[x] = xb[x];
         A.desc = st.d_desc.as<GroupDesc>();
         A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
         A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
