@@ -102,8 +102,10 @@ struct ShellTable {
 struct Timing {
     double gather_ms = 0, shell_ms = 0, peakfind_ms = 0;
     // persistent form, per stage: prepare (transpose, order, scan), k_gather,
-    // k_small, wave post, lane posts, finish
-    double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+    // k_small, wave post, lane post, finish, small-peak post (k_small and its
+    // post run on the side stream, beside the wave and lane posts)
+    static constexpr int NSTAGE = 7;
+    double stage_ms[NSTAGE] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 struct FftPlans {
@@ -118,6 +120,7 @@ struct FftPlans {
 struct pp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // second stream (default priority) for work beside the main stream's
     int num_cu = 0;
     size_t lds_per_block = 0;
     // collapse table
@@ -142,8 +145,8 @@ struct pp_ctx {
     int gather_waves_cu = 16;
     int post_width = 1;  // lanes per peak of the post below coop_m (1, 4, 8, 16)
     int lane_walk = 0;   // lane post: walks longer than this go to the wave post (0: no limit)
-    // events: [0,1] whole call, [2..8] persistent-form stage boundaries
-    static constexpr int NEV = 9;
+    // events: [0,1] whole call, [2..10] persistent-form stage boundaries
+    static constexpr int NEV = 11;
     hipEvent_t ev[NEV] = {};
     pp::Timing timing;
 };

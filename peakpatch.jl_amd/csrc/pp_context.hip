@@ -110,11 +110,16 @@ int pp_ctx_create(int device, pp_ctx **out)
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     for (int i = 0; i < pp_ctx::NEV && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) {
         set_error("context creation on device %d failed: %s", device, hipGetErrorString(e));
+        for (auto &ev : c->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (c->side) (void)hipStreamDestroy(c->side);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return PP_ERR_HIP;
     }
@@ -129,6 +134,7 @@ int pp_ctx_destroy(pp_ctx *ctx)
     if (!ctx) return PP_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     FftPlans &F = ctx->fft;
     if (F.r2c) rocfft_plan_destroy(F.r2c);
     if (F.c2r) rocfft_plan_destroy(F.c2r);
@@ -146,6 +152,7 @@ int pp_ctx_destroy(pp_ctx *ctx)
     for (auto &b : ctx->d_in) b.release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PP_OK;
@@ -156,6 +163,7 @@ int pp_ctx_synchronize(pp_ctx *ctx)
     PP_ARG(ctx, "ctx is NULL");
     PP_CHECK(ensure_device(ctx));
     PP_HIP(hipStreamSynchronize(ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->side));
     return PP_OK;
 }
 
@@ -299,7 +307,7 @@ int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get)
 int pp_last_shell_stages(pp_ctx *ctx, double *ms)
 {
     PP_ARG(ctx && ms, "NULL argument");
-    for (int q = 0; q < 6; q++) ms[q] = ctx->timing.stage_ms[q];
+    for (int q = 0; q < Timing::NSTAGE; q++) ms[q] = ctx->timing.stage_ms[q];
     return PP_OK;
 }
 

@@ -1,6 +1,9 @@
 // pp_persist.hip -- batched persistent form of the shell analysis.
 //
-// Pipeline for one pp_analyse_peaks call (all on the context stream):
+// Pipeline for one pp_analyse_peaks call (on the context stream; after the
+// wave gather the small peaks' gather and lane post run on the context's
+// second stream, beside the wave and lane posts and the finish pass, and are
+// joined back before the call returns):
 //   1. k_transpose: dT(x,y,z) = delta(z,x,y), so the reference's permuted read
 //      delta[iv3, iv1, iv2] (RadialShell.jl:448) coalesces like the others.
 //   2. peaks are ordered by finalised-shell count m, largest first (already
@@ -1218,7 +1221,10 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
         PP_HIP(hipGetLastError());
     }
+    // fork: the small peaks (own profile records, own result rows) are gathered
+    // and post-processed on the side stream, beside the posts below
     PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[4], 0));
     if (nblk > 0) {
         SmallArgs A;
         A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
@@ -1230,10 +1236,10 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
         int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
         grid = (grid + nx - 1) / nx * nx;
-        hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->stream, A);
+        hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->side, A);
         PP_HIP(hipGetLastError());
     }
-    PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
     // 5. posts
     if (coop_end > 0) {
         const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * 16));
@@ -1249,7 +1255,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         if (hi <= lo) return PP_OK;
         const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64, (int64_t)ctx->num_cu * 16));
         if (full) {
-            hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
+            hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->side, (const PeakParam *)pps, lo, hi,
                                d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0, 0x7fffffff, d_defer,
                                d_ndefer);
         } else {
@@ -1267,7 +1273,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipGetLastError());
         return PP_OK;
     };
-    PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
+    PP_CHECK(post_lane(small_lo, np, true));  // side stream
+    PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
+    PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
     if (ctx->post_width <= 1) {
         PP_CHECK(post_lane(coop_end, small_lo, false));
     } else if (small_lo > coop_end) {
@@ -1283,8 +1291,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         else launch(k_post_group<16>);
         PP_HIP(hipGetLastError());
     }
-    PP_CHECK(post_lane(small_lo, np, true));
-    PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     // 6. finish for the wave-gathered peaks
     if (small_lo > 0) {
         int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * 16));
@@ -1300,6 +1307,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         hipLaunchKernelGGL(k_finish, dim3(gf), dim3(64), 0, ctx->stream, A);
         PP_HIP(hipGetLastError());
     }
+    PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
+    // join: everything after this call on the context stream sees the small peaks' results
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[10], 0));
     if (cnt) {
         // profile export: counts and Fbar per shell in the caller's order
         std::vector<float> hr((size_t)tot * PR);
@@ -1331,15 +1341,15 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // stage times
     PP_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
     PP_HIP(hipEventSynchronize(ctx->ev[8]));
-    float t[6] = {0, 0, 0, 0, 0, 0};
-    (void)hipEventElapsedTime(&t[0], ctx->ev[2], ctx->ev[3]);
-    (void)hipEventElapsedTime(&t[1], ctx->ev[3], ctx->ev[4]);
-    (void)hipEventElapsedTime(&t[2], ctx->ev[4], ctx->ev[5]);
-    (void)hipEventElapsedTime(&t[3], ctx->ev[5], ctx->ev[6]);
-    (void)hipEventElapsedTime(&t[4], ctx->ev[6], ctx->ev[7]);
-    (void)hipEventElapsedTime(&t[5], ctx->ev[7], ctx->ev[8]);
-    for (int q = 0; q < 6; q++) ctx->timing.stage_ms[q] = t[q];
-    ctx->timing.gather_ms = t[1] + t[2];
+    // main stream: 2 start | prepare | 3 | k_gather | 4 (fork) | wave post | 5 |
+    // lane post | 6 | finish | 7 | join | 8;  side: 4 | k_small | 9 | small post | 10
+    const int from[Timing::NSTAGE] = {2, 3, 4, 4, 5, 6, 9}, to[Timing::NSTAGE] = {3, 4, 9, 5, 6, 7, 10};
+    for (int q = 0; q < Timing::NSTAGE; q++) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, ctx->ev[from[q]], ctx->ev[to[q]]);
+        ctx->timing.stage_ms[q] = t;
+    }
+    ctx->timing.gather_ms = ctx->timing.stage_ms[1] + ctx->timing.stage_ms[2];
     return PP_OK;
 }
 

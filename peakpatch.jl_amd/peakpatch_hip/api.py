@@ -144,12 +144,12 @@ class Context:
             L.check(self.lib.pp_shell_tunables(self.h, None, get.ctypes.data))
         return dict(zip(self.TUNABLES, (int(v) for v in get)))
 
-    STAGES = ("prepare", "gather", "small_gather", "post_wave", "post_lane", "finish")
+    STAGES = ("prepare", "gather", "small_gather", "post_wave", "post_lane", "finish", "small_post")
 
     def last_timing(self):
         g = C.c_double(); s = C.c_double(); p = C.c_double()
         L.check(self.lib.pp_last_timing(self.h, C.byref(g), C.byref(s), C.byref(p)))
-        st = (C.c_double * 6)()
+        st = (C.c_double * len(self.STAGES))()
         L.check(self.lib.pp_last_shell_stages(self.h, st))
         return {"gather_ms": g.value, "shell_ms": s.value, "peakfind_ms": p.value,
                 "stages_ms": dict(zip(self.STAGES, list(st)))}
