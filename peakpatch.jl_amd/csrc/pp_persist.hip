@@ -43,6 +43,15 @@
 namespace pp {
 
 constexpr int PR = 24;  // floats per shell record of the AoS profile
+// In memory the profile is two arrays of 12-float shell records: the strain
+// inputs G(3) Gf(3) E(6), which the posts' kernel_strain reads for every shell
+// of a band, and the rest (F, count, Fbar, S, S2, lapd), so the band loops
+// move 48 bytes per shell.  In registers a record is v[PR]: the strain part
+// in v[0..11], the rest in v[12..23].
+constexpr int PSR = 12;
+struct ProfBuf {
+    float *s, *a;  // record r: strain part at s + 12 r, the rest at a + 12 r
+};
 // record: G(3) Gf(3) E(6) | F, count | Fbar (double) | S(3) S2(3) lapd | pad;
 // the 12 strain inputs load as three 16-byte vectors.  S, S2, lapd are filled
 // for small peaks only.
@@ -211,10 +220,14 @@ __device__ __forceinline__ void group_sum(const GatherFields &F, const PeakParam
     }
 }
 
-__device__ __forceinline__ void store_record(float *r, const float v[PR], int nf4)
+// Record r: the strain part and the first naux float4 of the rest
+__device__ __forceinline__ void store_record(ProfBuf p, int64_t r, const float v[PR], int naux)
 {
-    float4 *o = (float4 *)r;
-    for (int q = 0; q < nf4; q++) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    float4 *os = (float4 *)(p.s + r * PSR), *oa = (float4 *)(p.a + r * PSR);
+#pragma unroll
+    for (int q = 0; q < 3; q++) os[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    for (int q = 0; q < naux; q++)
+        oa[q] = make_float4(v[PSR + 4 * q], v[PSR + 4 * q + 1], v[PSR + 4 * q + 2], v[PSR + 4 * q + 3]);
 }
 
 __device__ __forceinline__ void put_fbar(float v[PR], double fb)
@@ -295,7 +308,7 @@ struct SmallArgs {
     int32_t xp[NXCD + 1];
     int64_t xlen[NXCD];
     int nx;                    // lists (1 or NXCD); gridDim.x is a multiple of nx
-    float *prof;
+    ProfBuf prof;
     const int64_t *prof_off;
     const double *rad;
 };
@@ -396,7 +409,7 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
         __syncthreads();
         if (act) {
             put_fbar(v, sfb[lane]);
-            store_record(A.prof + (A.prof_off[idx] + (s - 1)) * PR, v, PR / 4);
+            store_record(A.prof, A.prof_off[idx] + (s - 1), v, 3);
         }
         __syncthreads();
     }
@@ -411,7 +424,7 @@ struct GatherArgs {
     const int32_t *full_nb;    //   and its number of 64-slot batches
     const int64_t *part_off;   // per m: the table of the last, partial chunk
     const int32_t *part_nb;
-    float *prof;
+    ProfBuf prof;
     const int64_t *prof_off;   // per sorted index: first shell record
     const double *rad;
     int *queue;                // nx counters, 64 ints apart
@@ -459,12 +472,12 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
         if (idx < 0) break;
         const PeakParam pk = A.pps[idx];
         const int m = pk.m;
-        float *rec = A.prof + A.prof_off[idx] * PR;
+        const int64_t rec = A.prof_off[idx];
         float F1;
         {
             float v[PR];
             F1 = center_values<false>(A.F, pk, v);
-            if (lane == 0) store_record(rec, v, 4);
+            if (lane == 0) store_record(A.prof, rec, v, 1);
         }
         double carry = 0.0, dFprev = (double)F1;
         const int nch = m >= 2 ? (m - 2) / cs + 1 : 0;
@@ -524,7 +537,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
                 const double Q = carry + wave_scan(term);
                 if (valid) {
                     put_fbar(v, Q / r3);
-                    store_record(rec + (int64_t)(s - 1) * PR, v, 4);
+                    store_record(A.prof, rec + (s - 1), v, 1);
                 }
                 const int last = min(63, ns - 1 - b0);
                 carry = __shfl(Q, last, 64);
@@ -538,16 +551,17 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
 // --------------------------------------------------------------- post
 template <bool FULL>
 struct AosProfile {
-    float *rec;
+    const float *sr, *ax;  // this peak's records (strain part, the rest)
     const double *radt;
     bool lapd;
-    __device__ const float *r(int s) const { return rec + (int64_t)(s - 1) * PR; }
+    __device__ const float *r(int s) const { return sr + (int64_t)(s - 1) * PSR; }
+    __device__ const float *x(int s) const { return ax + (int64_t)(s - 1) * PSR; }
     __device__ double rad(int s) const { return radt[s - 1]; }
-    __device__ double Fbar(int s) const { return *(const double *)(r(s) + P_FB); }
-    __device__ double count(int s) const { return (double)r(s)[P_N]; }
-    __device__ double S(int s, int q) const { return FULL ? (double)r(s)[P_S + q] : 0.0; }
-    __device__ double S2(int s, int q) const { return FULL ? (double)r(s)[P_S2 + q] : 0.0; }
-    __device__ double lap(int s) const { return FULL ? (double)r(s)[P_L] : 0.0; }
+    __device__ double Fbar(int s) const { return *(const double *)(x(s) + (P_FB - PSR)); }
+    __device__ double count(int s) const { return (double)x(s)[P_N - PSR]; }
+    __device__ double S(int s, int q) const { return FULL ? (double)x(s)[P_S - PSR + q] : 0.0; }
+    __device__ double S2(int s, int q) const { return FULL ? (double)x(s)[P_S2 - PSR + q] : 0.0; }
+    __device__ double lap(int s) const { return FULL ? (double)x(s)[P_L - PSR] : 0.0; }
     __device__ bool has_lapd() const { return FULL && lapd; }
     __device__ void strain_raw(int s, float G[3], float Gf[3], float Es[6]) const
     {
@@ -567,15 +581,15 @@ struct AosProfile {
 // its slowest lane) is handed to the wave post through the defer list.
 template <bool FULL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST_LANE_WAVES)))
-k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, float *__restrict__ prof,
+k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
             const int64_t *__restrict__ prof_off, PostConst C, ResultView R, int has_lap, int max_steps,
             int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += stride) {
         const PeakParam pk = pps[idx];
-        float *rec = prof + prof_off[idx] * PR;
-        AosProfile<FULL> P{rec, C.rad, has_lap != 0};
+        const int64_t rb = prof_off[idx];
+        AosProfile<FULL> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, has_lap != 0};
         if (analyse_post<1, FULL>(P, pk, C, R, pk.order, max_steps) == POST_DEFER) {
             defer[atomicAdd(ndefer, 1u)] = idx;
             continue;
@@ -586,28 +600,28 @@ k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, float *__
 
 // One wave per deferred peak.
 __global__ void __launch_bounds__(64) k_post_defer(const PeakParam *__restrict__ pps, const int64_t *__restrict__ defer,
-                                                   const unsigned int *__restrict__ ndefer, float *__restrict__ prof,
+                                                   const unsigned int *__restrict__ ndefer, ProfBuf prof,
                                                    const int64_t *__restrict__ prof_off, PostConst C, ResultView R)
 {
     const unsigned int n = *ndefer;
     for (unsigned int t = blockIdx.x; t < n; t += gridDim.x) {
         const int64_t idx = defer[t];
         const PeakParam pk = pps[idx];
-        float *rec = prof + prof_off[idx] * PR;
-        AosProfile<false> P{rec, C.rad, false};
+        const int64_t rb = prof_off[idx];
+        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, false};
         analyse_post<64, false>(P, pk, C, R, pk.order);
         if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
     }
 }
 
 __global__ void __launch_bounds__(64) k_post_wave(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
-                                                  float *__restrict__ prof, const int64_t *__restrict__ prof_off,
+                                                  ProfBuf prof, const int64_t *__restrict__ prof_off,
                                                   PostConst C, ResultView R)
 {
     for (int64_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
         const PeakParam pk = pps[idx];
-        float *rec = prof + prof_off[idx] * PR;
-        AosProfile<false> P{rec, C.rad, false};
+        const int64_t rb = prof_off[idx];
+        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, false};
         analyse_post<64, false>(P, pk, C, R, pk.order);
         if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
     }
@@ -616,7 +630,7 @@ __global__ void __launch_bounds__(64) k_post_wave(const PeakParam *__restrict__ 
 // W lanes per peak, 64 / W peaks per wave
 template <int W>
 __global__ void __launch_bounds__(64) k_post_group(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
-                                                   float *__restrict__ prof, const int64_t *__restrict__ prof_off,
+                                                   ProfBuf prof, const int64_t *__restrict__ prof_off,
                                                    PostConst C, ResultView R)
 {
     constexpr int G = 64 / W;
@@ -625,8 +639,8 @@ __global__ void __launch_bounds__(64) k_post_group(const PeakParam *__restrict__
         const int64_t idx = b0 + grp;
         if (idx >= hi) continue;  // whole lane groups drop out
         const PeakParam pk = pps[idx];
-        float *rec = prof + prof_off[idx] * PR;
-        AosProfile<false> P{rec, C.rad, false};
+        const int64_t rb = prof_off[idx];
+        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, false};
         analyse_post<W, false>(P, pk, C, R, pk.order);
         if ((threadIdx.x & (W - 1)) == 0) R.nshells[pk.order] = pk.m;
     }
@@ -1105,7 +1119,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         tot = last[0] + last[1];
     }
     PP_CHECK(ctx->d_res.grow((size_t)std::max<int64_t>(tot, 1) * PR * sizeof(float) + 4096));
-    float *d_rec = ctx->d_res.as<float>();
+    const ProfBuf d_rec{ctx->d_res.as<float>(), ctx->d_res.as<float>() + (size_t)std::max<int64_t>(tot, 1) * PSR};
     PP_CHECK(ctx->d_misc.grow(8192));
     int *d_q = (int *)(ctx->d_misc.as<char>() + 4096);  // NXCD queue counters, 256 B apart
     // per-XCD lists of the wave-gathered peaks [0, small_lo): part x of every
@@ -1312,8 +1326,8 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[10], 0));
     if (cnt) {
         // profile export: counts and Fbar per shell in the caller's order
-        std::vector<float> hr((size_t)tot * PR);
-        PP_HIP(hipMemcpyAsync(hr.data(), d_rec, hr.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<float> hr((size_t)tot * PSR);
+        PP_HIP(hipMemcpyAsync(hr.data(), d_rec.a, hr.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
         std::vector<PeakParam> hpp(np);
         PP_HIP(hipMemcpyAsync(hpp.data(), pps, np * sizeof(PeakParam), hipMemcpyDeviceToHost, ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));
@@ -1329,10 +1343,10 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
             const int64_t so = start_of[hpp[i].order];
             (*offs)[hpp[i].order] = so;
             for (int s = 0; s < m; s++) {
-                const float *r = &hr[(size_t)(o2 + s) * PR];
-                (*cnt)[so + s] = (int32_t)r[P_N];
+                const float *r = &hr[(size_t)(o2 + s) * PSR];
+                (*cnt)[so + s] = (int32_t)r[P_N - PSR];
                 double fb;
-                memcpy(&fb, r + P_FB, 8);
+                memcpy(&fb, r + (P_FB - PSR), 8);
                 (*fbar)[so + s] = fb;
             }
             o2 += m;
