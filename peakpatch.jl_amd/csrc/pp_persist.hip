@@ -1213,54 +1213,6 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipMemcpyAsync(st.d_seg.as<char>() + seg_bytes, st.parts_host.data(),
                               st.parts_host.size() * sizeof(SmallPart), hipMemcpyHostToDevice, ctx->stream));
     }
-    PP_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
-    // 4. gathers
-    if (small_lo > 0) {
-        const size_t lds = ((size_t)cs * 2 + 64) * P_NB * sizeof(float);
-        const size_t lds_max = std::min<size_t>(ctx->lds_per_block, 160 * 1024);
-        const int waves_cu = std::max(1, std::min<int>(ctx->gather_waves_cu > 0 ? ctx->gather_waves_cu : 16,
-                                                       (int)(lds_max / lds)));
-        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * waves_cu));
-        hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(256), 0, ctx->stream, d_q, (int64_t)(64 * NXCD), 0);
-        GatherArgs A;
-        A.F = G; A.pps = pps;
-        A.desc = st.d_desc.as<GroupDesc>();
-        A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
-        A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
-        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs;
-        A.xorder = W.xg; A.nx = nx;
-        for (int x = 0; x <= NXCD; x++) A.xb[x] = xb[x];
-        if (lds > 64 * 1024)
-            (void)hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
-        PP_HIP(hipGetLastError());
-    }
-    // fork: the small peaks (own profile records, own result rows) are gathered
-    // and post-processed on the side stream, beside the posts below
-    PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
-    PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[4], 0));
-    if (nblk > 0) {
-        SmallArgs A;
-        A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
-        A.seg = st.d_seg.as<SmallSeg>();
-        A.part = (const SmallPart *)(st.d_seg.as<char>() + seg_bytes);
-        for (int x = 0; x <= NXCD; x++) A.xp[x] = sxp[x];
-        for (int x = 0; x < NXCD; x++) A.xlen[x] = sxlen[x];
-        A.nx = nx;
-        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
-        int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
-        grid = (grid + nx - 1) / nx * nx;
-        hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->side, A);
-        PP_HIP(hipGetLastError());
-    }
-    PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
-    // 5. posts
-    if (coop_end > 0) {
-        const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * 16));
-        hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps, (int64_t)0, coop_end,
-                           d_rec, (const int64_t *)d_off, C, S.R);
-        PP_HIP(hipGetLastError());
-    }
     // defer list of the lane post: [count | pad | indices]
     PP_CHECK(ctx->d_k2.grow(256 + (size_t)std::max<int64_t>(small_lo - coop_end, 1) * sizeof(int64_t)));
     unsigned int *d_ndefer = ctx->d_k2.as<unsigned int>();
@@ -1287,8 +1239,62 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipGetLastError());
         return PP_OK;
     };
-    PP_CHECK(post_lane(small_lo, np, true));  // side stream
-    PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
+    // The small peaks (own profile records, own result rows) are gathered and
+    // post-processed on the side stream, beside the main stream's work from
+    // the fork on; the main stream joins them before the call returns.
+    auto side_work = [&]() -> int {
+        PP_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
+        PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[11], 0));
+        if (nblk > 0) {
+            SmallArgs A;
+            A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
+            A.seg = st.d_seg.as<SmallSeg>();
+            A.part = (const SmallPart *)(st.d_seg.as<char>() + seg_bytes);
+            for (int x = 0; x <= NXCD; x++) A.xp[x] = sxp[x];
+            for (int x = 0; x < NXCD; x++) A.xlen[x] = sxlen[x];
+            A.nx = nx;
+            A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
+            int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
+            grid = (grid + nx - 1) / nx * nx;
+            hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->side, A);
+            PP_HIP(hipGetLastError());
+        }
+        PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
+        PP_CHECK(post_lane(small_lo, np, true));
+        PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
+        return PP_OK;
+    };
+    PP_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+    // 4. gathers
+    if (small_lo > 0) {
+        const size_t lds = ((size_t)cs * 2 + 64) * P_NB * sizeof(float);
+        const size_t lds_max = std::min<size_t>(ctx->lds_per_block, 160 * 1024);
+        const int waves_cu = std::max(1, std::min<int>(ctx->gather_waves_cu > 0 ? ctx->gather_waves_cu : 16,
+                                                       (int)(lds_max / lds)));
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * waves_cu));
+        hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(256), 0, ctx->stream, d_q, (int64_t)(64 * NXCD), 0);
+        GatherArgs A;
+        A.F = G; A.pps = pps;
+        A.desc = st.d_desc.as<GroupDesc>();
+        A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
+        A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
+        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs;
+        A.xorder = W.xg; A.nx = nx;
+        for (int x = 0; x <= NXCD; x++) A.xb[x] = xb[x];
+        if (lds > 64 * 1024)
+            (void)hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
+        PP_HIP(hipGetLastError());
+    }
+    PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    PP_CHECK(side_work());  // fork after the wave gather (measured best: beside the posts, not the gather)
+    // 5. posts
+    if (coop_end > 0) {
+        const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * 16));
+        hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps, (int64_t)0, coop_end,
+                           d_rec, (const int64_t *)d_off, C, S.R);
+        PP_HIP(hipGetLastError());
+    }
     PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
     if (ctx->post_width <= 1) {
         PP_CHECK(post_lane(coop_end, small_lo, false));
@@ -1355,9 +1361,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // stage times
     PP_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
     PP_HIP(hipEventSynchronize(ctx->ev[8]));
-    // main stream: 2 start | prepare | 3 | k_gather | 4 (fork) | wave post | 5 |
-    // lane post | 6 | finish | 7 | join | 8;  side: 4 | k_small | 9 | small post | 10
-    const int from[Timing::NSTAGE] = {2, 3, 4, 4, 5, 6, 9}, to[Timing::NSTAGE] = {3, 4, 9, 5, 6, 7, 10};
+    // main stream: 2 start | prepare | 3 | k_gather | 4 | wave post | 5 | lane post |
+    // 6 | finish | 7 | join | 8;  side: 11 (fork) | k_small | 9 | small post | 10
+    const int from[Timing::NSTAGE] = {2, 3, 11, 4, 5, 6, 9}, to[Timing::NSTAGE] = {3, 4, 9, 5, 6, 7, 10};
     for (int q = 0; q < Timing::NSTAGE; q++) {
         float t = 0.0f;
         (void)hipEventElapsedTime(&t, ctx->ev[from[q]], ctx->ev[to[q]]);

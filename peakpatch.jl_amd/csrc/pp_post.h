@@ -496,7 +496,10 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         double Frho_m0, e_v_m0, p_v_m0;
         evals_ep(s0.E, Frho_m0, e_v_m0, p_v_m0);
         double zvir1 = -1.0, zvir1p = -1.0;
-        Strain last = s0;               // Ebar_last, gradpk_last, gradpkf_last
+        // Ebar_last, gradpk_last, gradpkf_last: set at the collapse step only
+        // (between steps it would equal the previous step's, kept below), so
+        // it is not live across the walk
+        Strain last = s0;
         double strain_mat[6];           // the previous step's Ebar
         double gradpk_prev[3], gradpkf_prev[3];
         double Frhoh = Frho_m0, Frho = P.Fbar(m0), e_v = e_v_m0, p_v = p_v_m0;
@@ -544,7 +547,6 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 for (int L = 0; L < 3; L++) { gradpk_prev[L] = smp.g[L]; gradpkf_prev[L] = smp.gf[L]; }
                 for (int i = 0; i < 6; i++) strain_mat[i] = smp.E[i];
                 Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
-                last = smp;
             }
         }
         (void)Frho;
