@@ -4,7 +4,8 @@
 // wave gather the small peaks' gather and lane post run on the context's
 // second stream, beside the wave and lane posts and the finish pass, and are
 // joined back before the call returns):
-//   1. k_transpose: dT(x,y,z) = delta(z,x,y), so the reference's permuted read
+//   1. k_pack4: (delta, dT, eta x, eta y) per cell as one float4, with
+//      dT(x,y,z) = delta(z,x,y), so the reference's permuted read
 //      delta[iv3, iv1, iv2] (RadialShell.jl:448) coalesces like the others.
 //   2. peaks are ordered by finalised-shell count m, largest first (already
 //      the case for the peak finder's output, else a radix sort), and every
@@ -69,8 +70,12 @@ constexpr int NXCD = 8;
 #define PP_LOCALITY 2  // 0: one list in m order, 1: per-XCD lists, 2: + Morton order in the lists
 #endif
 
-// ------------------------------------------------------------- transpose
-__global__ void __launch_bounds__(256) k_transpose(const float *__restrict__ in, float *__restrict__ out, int n)
+// (delta, delta transposed, eta x, eta y) of every cell as one float4, so the
+// gathers read the four with one 16-byte load (and a 4-byte one for eta z):
+// a sphere row then spans about half the cache lines of five separate arrays.
+// Tiled like k_transpose: the block stages delta(z, x, y) for 32 x 32 (x, z).
+__global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, const float *__restrict__ ex,
+                                               const float *__restrict__ ey, float4 *__restrict__ out, int n)
 {
     __shared__ float tile[32][33];
     const int y = blockIdx.z, x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
@@ -78,12 +83,15 @@ __global__ void __launch_bounds__(256) k_transpose(const float *__restrict__ in,
     const long nn = (long)n * n;
     for (int r = ty; r < 32; r += 8) {
         const int x = x0 + r, z = z0 + tx;
-        if (x < n && z < n) tile[r][tx] = in[z + (long)x * n + (long)y * nn];
+        if (x < n && z < n) tile[r][tx] = d[z + (long)x * n + (long)y * nn];
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         const int z = z0 + r, x = x0 + tx;
-        if (x < n && z < n) out[x + (long)y * n + (long)z * nn] = tile[tx][r];
+        if (x < n && z < n) {
+            const long a = x + (long)y * n + (long)z * nn;
+            out[a] = make_float4(d[a], tile[tx][r], ex[a], ey[a]);
+        }
     }
 }
 
@@ -200,7 +208,8 @@ __device__ __forceinline__ void group_sum(const GatherFields &F, const PeakParam
         const int x = pk.ci + dx, y = pk.cj + dy, z = pk.ck + dz;
         if (!inside && (x < 0 || x >= F.n || y < 0 || y >= F.n || z < 0 || z >= F.n)) continue;
         const long a = x + y * n1 + z * n12;
-        const float d = F.d[a], dT = F.dT[a], ex = F.ex[a], ey = F.ey[a], ez = F.ez[a];
+        const float4 q = F.p4[a];
+        const float d = q.x, dT = q.y, ex = q.z, ey = q.w, ez = F.ez[a];
         const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
         acc[P_F] += d;
         acc[P_N] += 1.0f;
@@ -347,7 +356,8 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
                 const int x = pk.ci + dx, y = pk.cj + dy, z = pk.ck + dz;
                 if ((pk.flags & 1) || (x >= 0 && x < F.n && y >= 0 && y < F.n && z >= 0 && z < F.n)) {
                     const long a = x + y * n1 + z * n12;
-                    const float d = F.d[a], dT = F.dT[a], ex = F.ex[a], ey = F.ey[a], ez = F.ez[a];
+                    const float4 q = F.p4[a];
+                    const float d = q.x, dT = q.y, ex = q.z, ey = q.w, ez = F.ez[a];
                     const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
                     acc[P_G + 0] = d * fx; acc[P_G + 1] = d * fy; acc[P_G + 2] = d * fz;
                     acc[P_GF + 0] = dT * fx; acc[P_GF + 1] = dT * fy; acc[P_GF + 2] = dT * fz;
@@ -1069,13 +1079,14 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     if (np > INT_MAX) { set_error("persistent form: at most 2^31-1 peaks per call"); return PP_ERR_CAPACITY; }
     for (double &t : ctx->timing.stage_ms) t = 0.0;
     PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    // 1. transposed delta
-    PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float)));
+    // 1. packed (delta, delta transposed, eta x, eta y)
+    PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4)));
     {
         dim3 grid((n + 31) / 32, (n + 31) / 32, n), block(32, 8);
-        hipLaunchKernelGGL(k_transpose, grid, block, 0, ctx->stream, G.d, ctx->d_field_b.as<float>(), n);
+        hipLaunchKernelGGL(k_pack4, grid, block, 0, ctx->stream, G.d, G.ex, G.ey, ctx->d_field_b.as<float4>(), n);
         PP_HIP(hipGetLastError());
-        G.dT = ctx->d_field_b.as<float>();
+        G.p4 = ctx->d_field_b.as<float4>();
+        G.dT = nullptr;
     }
     // 2. order by m (descending)
     PeakParam *pps = nullptr;

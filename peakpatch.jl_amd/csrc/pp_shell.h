@@ -14,6 +14,7 @@ namespace pp {
 struct GatherFields {
     const float *d, *ex, *ey, *ez, *e2x, *e2y, *e2z, *lap;
     const float *dT;
+    const float4 *p4;  // persistent form: (delta, delta transposed, eta x, eta y) per cell, or null
     int n;
 };
 

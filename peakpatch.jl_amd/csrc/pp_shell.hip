@@ -386,6 +386,7 @@ int upload_fields(pp_ctx *ctx, const pp_fields *f, GatherFields &G)
     G.d = dst[0]; G.ex = dst[1]; G.ey = dst[2]; G.ez = dst[3];
     G.e2x = dst[4]; G.e2y = dst[5]; G.e2z = dst[6]; G.lap = dst[7];
     G.dT = nullptr;
+    G.p4 = nullptr;
     G.n = n;
     return PP_OK;
 }
