@@ -73,9 +73,14 @@ constexpr int NXCD = 8;
 // (delta, delta transposed, eta x, eta y) of every cell as one float4, so the
 // gathers read the four with one 16-byte load (and a 4-byte one for eta z):
 // a sphere row then spans about half the cache lines of five separate arrays.
-// Tiled like k_transpose: the block stages delta(z, x, y) for 32 x 32 (x, z).
+// With eta2, (eta z, eta2 x, eta2 y, eta2 z) go to a second float4 array
+// (q4) for the small-peak gather and the finish pass.  The block stages
+// delta(z, x, y) for 32 x 32 (x, z) in LDS (the transposed read).
 __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, const float *__restrict__ ex,
-                                               const float *__restrict__ ey, float4 *__restrict__ out, int n)
+                                               const float *__restrict__ ey, const float *__restrict__ ez,
+                                               const float *__restrict__ e2x, const float *__restrict__ e2y,
+                                               const float *__restrict__ e2z, float4 *__restrict__ out,
+                                               float4 *__restrict__ q4, int n)
 {
     __shared__ float tile[32][33];
     const int y = blockIdx.z, x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
@@ -91,6 +96,7 @@ __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, cons
         if (x < n && z < n) {
             const long a = x + (long)y * n + (long)z * nn;
             out[a] = make_float4(d[a], tile[tx][r], ex[a], ey[a]);
+            if (q4) q4[a] = make_float4(ez[a], e2x[a], e2y[a], e2z[a]);
         }
     }
 }
@@ -357,7 +363,11 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
                 if ((pk.flags & 1) || (x >= 0 && x < F.n && y >= 0 && y < F.n && z >= 0 && z < F.n)) {
                     const long a = x + y * n1 + z * n12;
                     const float4 q = F.p4[a];
-                    const float d = q.x, dT = q.y, ex = q.z, ey = q.w, ez = F.ez[a];
+                    float4 q2;
+                    if (F.q4) q2 = F.q4[a];
+                    else q2 = make_float4(F.ez[a], F.e2x ? F.e2x[a] : 0.0f, F.e2x ? F.e2y[a] : 0.0f,
+                                          F.e2x ? F.e2z[a] : 0.0f);
+                    const float d = q.x, dT = q.y, ex = q.z, ey = q.w, ez = q2.x;
                     const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
                     acc[P_G + 0] = d * fx; acc[P_G + 1] = d * fy; acc[P_G + 2] = d * fz;
                     acc[P_GF + 0] = dT * fx; acc[P_GF + 1] = dT * fy; acc[P_GF + 2] = dT * fz;
@@ -370,7 +380,7 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
                     acc[P_F] = d;
                     acc[P_N] = 1.0f;
                     acc[P_NB + 0] = ex; acc[P_NB + 1] = ey; acc[P_NB + 2] = ez;
-                    if (F.e2x) { acc[P_NB + 3] = F.e2x[a]; acc[P_NB + 4] = F.e2y[a]; acc[P_NB + 5] = F.e2z[a]; }
+                    if (F.e2x) { acc[P_NB + 3] = q2.y; acc[P_NB + 4] = q2.z; acc[P_NB + 5] = q2.w; }
                     if (F.lap) acc[P_NB + 6] = F.lap[a];
                 }
             }
@@ -583,6 +593,9 @@ struct AosProfile {
     }
 };
 
+#ifndef PP_PACK_Q4
+#define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
+#endif
 #ifndef PP_POST_LANE_WAVES
 #define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
 #endif
@@ -740,9 +753,18 @@ __global__ void __launch_bounds__(64) k_finish(FinishArgs A)
                     }
                     const long a = x + y * n1 + z * n12;
                     if (inS) {
-                        for (int L = 0; L < 3; L++) {
-                            Sb[L] += (double)ex[L][a];
-                            if (e2[0]) Sb2[L] += (double)e2[L][a];
+                        if (F.q4) {
+                            const float4 p = F.p4[a], q2 = F.q4[a];
+                            const float ev[3] = {p.z, p.w, q2.x}, e2v[3] = {q2.y, q2.z, q2.w};
+                            for (int L = 0; L < 3; L++) {
+                                Sb[L] += (double)ev[L];
+                                Sb2[L] += (double)e2v[L];
+                            }
+                        } else {
+                            for (int L = 0; L < 3; L++) {
+                                Sb[L] += (double)ex[L][a];
+                                if (e2[0]) Sb2[L] += (double)e2[L][a];
+                            }
                         }
                         nS += 1.0;
                     }
@@ -1080,12 +1102,16 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     for (double &t : ctx->timing.stage_ms) t = 0.0;
     PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     // 1. packed (delta, delta transposed, eta x, eta y)
-    PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4)));
+    const bool pack_q4 = PP_PACK_Q4 && G.e2x != nullptr;
+    PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4) * (pack_q4 ? 2 : 1)));
     {
         dim3 grid((n + 31) / 32, (n + 31) / 32, n), block(32, 8);
-        hipLaunchKernelGGL(k_pack4, grid, block, 0, ctx->stream, G.d, G.ex, G.ey, ctx->d_field_b.as<float4>(), n);
+        float4 *q4 = pack_q4 ? ctx->d_field_b.as<float4>() + (size_t)n * n * n : nullptr;
+        hipLaunchKernelGGL(k_pack4, grid, block, 0, ctx->stream, G.d, G.ex, G.ey, G.ez, G.e2x, G.e2y, G.e2z,
+                           ctx->d_field_b.as<float4>(), q4, n);
         PP_HIP(hipGetLastError());
         G.p4 = ctx->d_field_b.as<float4>();
+        G.q4 = q4;
         G.dT = nullptr;
     }
     // 2. order by m (descending)

@@ -15,6 +15,7 @@ struct GatherFields {
     const float *d, *ex, *ey, *ez, *e2x, *e2y, *e2z, *lap;
     const float *dT;
     const float4 *p4;  // persistent form: (delta, delta transposed, eta x, eta y) per cell, or null
+    const float4 *q4;  // persistent form with eta2: (eta z, eta2 x, eta2 y, eta2 z) per cell, or null
     int n;
 };
 

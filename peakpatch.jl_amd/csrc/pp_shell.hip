@@ -387,6 +387,7 @@ int upload_fields(pp_ctx *ctx, const pp_fields *f, GatherFields &G)
     G.e2x = dst[4]; G.e2y = dst[5]; G.e2z = dst[6]; G.lap = dst[7];
     G.dT = nullptr;
     G.p4 = nullptr;
+    G.q4 = nullptr;
     G.n = n;
     return PP_OK;
 }
