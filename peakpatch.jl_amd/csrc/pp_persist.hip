@@ -1101,15 +1101,19 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     if (np > INT_MAX) { set_error("persistent form: at most 2^31-1 peaks per call"); return PP_ERR_CAPACITY; }
     for (double &t : ctx->timing.stage_ms) t = 0.0;
     PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    // 1. packed (delta, delta transposed, eta x, eta y)
+    // 1. packed (delta, delta transposed, eta x, eta y) [and (eta z, eta2)], on
+    // the side stream while the main stream orders the peaks; joined before
+    // the gathers
     const bool pack_q4 = PP_PACK_Q4 && G.e2x != nullptr;
     PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4) * (pack_q4 ? 2 : 1)));
     {
+        PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[2], 0));
         dim3 grid((n + 31) / 32, (n + 31) / 32, n), block(32, 8);
         float4 *q4 = pack_q4 ? ctx->d_field_b.as<float4>() + (size_t)n * n * n : nullptr;
-        hipLaunchKernelGGL(k_pack4, grid, block, 0, ctx->stream, G.d, G.ex, G.ey, G.ez, G.e2x, G.e2y, G.e2z,
+        hipLaunchKernelGGL(k_pack4, grid, block, 0, ctx->side, G.d, G.ex, G.ey, G.ez, G.e2x, G.e2y, G.e2z,
                            ctx->d_field_b.as<float4>(), q4, n);
         PP_HIP(hipGetLastError());
+        PP_HIP(hipEventRecord(ctx->ev[12], ctx->side));
         G.p4 = ctx->d_field_b.as<float4>();
         G.q4 = q4;
         G.dT = nullptr;
@@ -1301,6 +1305,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
         return PP_OK;
     };
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[12], 0));  // the packed fields
     PP_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     // 4. gathers
     if (small_lo > 0) {
