@@ -452,6 +452,7 @@ struct GatherArgs {
     int64_t xb[NXCD + 1];      // list x = xorder[xb[x], xb[x + 1])
     int nx;                    // lists (1 or NXCD)
     int cs;                    // shells per chunk
+    int steps;                 // shuffle-tree steps: 2^steps >= the longest run of equal shells in a batch
 };
 
 __device__ __forceinline__ void neumaier(float *hi, float *lo, float x)
@@ -463,10 +464,10 @@ __device__ __forceinline__ void neumaier(float *hi, float *lo, float x)
     *lo += e;
 }
 
-// One wave per peak.  Per batch: each lane sums its group's images, the 14
-// partial sums go to an LDS staging row, and the first lane of each run of
-// equal shells (the host sorted every batch by shell) adds its run in lane
-// order and deposits it into the chunk's bins.  The records of a chunk are
+// One wave per peak.  Per batch: each lane sums its group's images, the
+// lanes of each run of equal shells (the host sorted every batch by shell)
+// sum their 14 partial sums by a segmented tree over wave shuffles, and the
+// run's first lane deposits the sum into the chunk's bins.  The records of a chunk are
 // written with Fbar, whose recursion is carried as the prefix sum
 // Q(s) = Fbar(s) r^3(s) = sum_{t=2..s} (dF(t-1) + dF(t)) / 2 (r^3(t) - r^3(t-1)).
 __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
@@ -474,8 +475,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
     extern __shared__ float lds[];
     const int lane = threadIdx.x;
     const int cs = A.cs;
-    float *stg = lds;                  // [P_NB][64]
-    float *bh = lds + P_NB * 64, *bl = bh + P_NB * cs;
+    float *bh = lds, *bl = bh + P_NB * cs;  // [P_NB][cs] each
     // own XCD's list first, then the others' (the tail of the work)
     const int x0 = blockIdx.x % A.nx;
     int xq = 0;
@@ -515,23 +515,21 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
 #pragma unroll
                 for (int f = 0; f < PR; f++) acc[f] = 0.0f;
                 if (g.s >= 2) group_sum<false>(A.F, pk, g, acc);
+                const int rend = (g.lmn >> 24) & 127;
+                for (int k = 0; k < A.steps; k++) {
+                    const int d = 1 << k;
+                    const bool in = lane + d < rend;
 #pragma unroll
-                for (int f = 0; f < P_NB; f++) stg[f * 64 + lane] = acc[f];
-                __syncthreads();
-                const int len = (g.lmn >> 24) & 127;
-                if (len > 0) {
-                    float v[P_NB];
-#pragma unroll
-                    for (int f = 0; f < P_NB; f++) v[f] = stg[f * 64 + lane];
-                    for (int t = 1; t < len; t++) {
-#pragma unroll
-                        for (int f = 0; f < P_NB; f++) v[f] += stg[f * 64 + lane + t];
+                    for (int f = 0; f < P_NB; f++) {
+                        const float u = __shfl_down(acc[f], d, 64);
+                        if (in) acc[f] += u;
                     }
+                }
+                if (g.lmn < 0) {  // run head
                     const int sb = g.s - s0;
 #pragma unroll
-                    for (int f = 0; f < P_NB; f++) neumaier(&bh[f * cs + sb], &bl[f * cs + sb], v[f]);
+                    for (int f = 0; f < P_NB; f++) neumaier(&bh[f * cs + sb], &bl[f * cs + sb], acc[f]);
                 }
-                __syncthreads();
             }
             for (int b0 = 0; b0 < ns; b0 += 64) {
                 const int b = b0 + lane;
@@ -810,6 +808,7 @@ struct PersistState {
     std::vector<CellSlot> scells;                     // small-peak batch tables
     std::vector<SmallTab> stab;                       // index m (0..small_max)
     int small_max = 0;
+    int gsteps = 0;                                   // shuffle-tree steps of the gather tables
     std::vector<SmallSeg> segs_host;
     std::vector<SmallPart> parts_host;                // per-XCD block lists of k_small
     std::vector<int64_t> xtab_host;                   // per m: k_xcd_order's table row
@@ -871,10 +870,11 @@ static void for_groups(const ShellTable &T, const std::vector<int32_t> &shell_of
 
 // Groups of shells [slo, shi] in row order cut into consecutive 64-slot
 // batches (neighbouring x-runs share cache lines); inside a batch the
-// groups are sorted by shell and the first lane of each run of equal shells
-// carries the run length in lmn bits 24-30 (its deposit lane).
+// groups are sorted by shell; every lane carries the end of its run of equal
+// shells in lmn bits 24-30 and the first lane of a run (its deposit lane)
+// bit 31.  steps grows to ceil(log2) of the longest run.
 static void pack_groups(const ShellTable &T, const std::vector<int32_t> &shell_of_r2, int slo, int shi,
-                        std::vector<GroupDesc> &out, int &nbatch)
+                        std::vector<GroupDesc> &out, int &nbatch, int &steps)
 {
     std::vector<GroupDesc> g;
     for_groups(T, shell_of_r2, slo, shi, [&](GroupDesc d) { g.push_back(d); });
@@ -888,7 +888,9 @@ static void pack_groups(const ShellTable &T, const std::vector<int32_t> &shell_o
         for (size_t q = lo; q < hi;) {
             size_t r = q;
             while (r < hi && g[r].s == g[q].s) r++;
-            g[q].lmn |= (int32_t)(r - q) << 24;
+            for (size_t u = q; u < r; u++) g[u].lmn |= (int32_t)(r - lo) << 24;  // run end (lane)
+            g[q].lmn |= (int32_t)(1u << 31);                                      // run head
+            while ((1 << steps) < (int)(r - q)) steps++;
             q = r;
         }
         for (size_t q = lo; q < hi; q++) out[base + q] = g[q];
@@ -973,7 +975,7 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
             const int c = (int)st.full.size();
             int nb = 0;
             const int64_t off = (int64_t)st.desc.size();
-            pack_groups(T, st.shell_of_r2, 2 + c * cs, 1 + (c + 1) * cs, st.desc, nb);
+            pack_groups(T, st.shell_of_r2, 2 + c * cs, 1 + (c + 1) * cs, st.desc, nb, st.gsteps);
             st.full.push_back({off, nb});
             st.dirty = true;
         }
@@ -981,7 +983,7 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
             const int c = (m - 2) / cs;
             int nb = 0;
             const int64_t off = (int64_t)st.desc.size();
-            pack_groups(T, st.shell_of_r2, 2 + c * cs, m, st.desc, nb);
+            pack_groups(T, st.shell_of_r2, 2 + c * cs, m, st.desc, nb, st.gsteps);
             st.part[m] = {off, nb};
             st.dirty = true;
         }
@@ -1309,7 +1311,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     // 4. gathers
     if (small_lo > 0) {
-        const size_t lds = ((size_t)cs * 2 + 64) * P_NB * sizeof(float);
+        const size_t lds = (size_t)cs * 2 * P_NB * sizeof(float);
         const size_t lds_max = std::min<size_t>(ctx->lds_per_block, 160 * 1024);
         const int waves_cu = std::max(1, std::min<int>(ctx->gather_waves_cu > 0 ? ctx->gather_waves_cu : 16,
                                                        (int)(lds_max / lds)));
@@ -1320,7 +1322,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         A.desc = st.d_desc.as<GroupDesc>();
         A.full_off = st.d_full_off.as<int64_t>(); A.full_nb = st.d_full_nb.as<int32_t>();
         A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
-        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs;
+        A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs; A.steps = st.gsteps;
         A.xorder = W.xg; A.nx = nx;
         for (int x = 0; x <= NXCD; x++) A.xb[x] = xb[x];
         if (lds > 64 * 1024)
