@@ -594,6 +594,9 @@ struct AosProfile {
 #ifndef PP_PACK_Q4
 #define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
 #endif
+#ifndef PP_LANE_GRID_CU
+#define PP_LANE_GRID_CU 64  // waves per CU launched for the lane posts (grid-stride; more, finer rounds balance better)
+#endif
 #ifndef PP_POST_LANE_WAVES
 #define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
 #endif
@@ -1262,7 +1265,8 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     int64_t *d_defer = (int64_t *)(ctx->d_k2.as<char>() + 256);
     auto post_lane = [&](int64_t lo, int64_t hi, bool full) -> int {
         if (hi <= lo) return PP_OK;
-        const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64, (int64_t)ctx->num_cu * 16));
+        const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64,
+                                                                   (int64_t)ctx->num_cu * PP_LANE_GRID_CU));
         if (full) {
             hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->side, (const PeakParam *)pps, lo, hi,
                                d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0, 0x7fffffff, d_defer,
