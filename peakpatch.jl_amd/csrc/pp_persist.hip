@@ -594,6 +594,15 @@ struct AosProfile {
 #ifndef PP_PACK_Q4
 #define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
 #endif
+#ifndef PP_SMALL_GRID_CU
+#define PP_SMALL_GRID_CU 32  // workgroups per CU launched for k_small (grid-stride)
+#endif
+#ifndef PP_WAVE_GRID_CU
+#define PP_WAVE_GRID_CU 16  // waves per CU launched for the wave post (grid-stride)
+#endif
+#ifndef PP_FINISH_GRID_CU
+#define PP_FINISH_GRID_CU 64  // waves per CU launched for the finish pass (strided over its lists)
+#endif
 #ifndef PP_LANE_GRID_CU
 #define PP_LANE_GRID_CU 64  // waves per CU launched for the lane posts (grid-stride; more, finer rounds balance better)
 #endif
@@ -1301,7 +1310,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
             for (int x = 0; x < NXCD; x++) A.xlen[x] = sxlen[x];
             A.nx = nx;
             A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
-            int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * 32));
+            int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * PP_SMALL_GRID_CU));
             grid = (grid + nx - 1) / nx * nx;
             hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->side, A);
             PP_HIP(hipGetLastError());
@@ -1338,7 +1347,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_CHECK(side_work());  // fork after the wave gather (measured best: beside the posts, not the gather)
     // 5. posts
     if (coop_end > 0) {
-        const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * 16));
+        const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * PP_WAVE_GRID_CU));
         hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps, (int64_t)0, coop_end,
                            d_rec, (const int64_t *)d_off, C, S.R);
         PP_HIP(hipGetLastError());
@@ -1362,7 +1371,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     // 6. finish for the wave-gathered peaks
     if (small_lo > 0) {
-        int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * 16));
+        int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * PP_FINISH_GRID_CU));
         gf = (gf + nx - 1) / nx * nx;
         FinishArgs A;
         A.F = G; A.pps = pps;
