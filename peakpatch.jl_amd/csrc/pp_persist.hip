@@ -594,6 +594,9 @@ struct AosProfile {
 #ifndef PP_PACK_Q4
 #define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
 #endif
+#ifndef PP_WAVE_ON_SIDE
+#define PP_WAVE_ON_SIDE 1  // the wave post runs on the side stream after the small peaks (beside the lane post)
+#endif
 #ifndef PP_SMALL_GRID_CU
 #define PP_SMALL_GRID_CU 32  // workgroups per CU launched for k_small (grid-stride)
 #endif
@@ -1317,7 +1320,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         }
         PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
         PP_CHECK(post_lane(small_lo, np, true));
-        PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
+        if (!PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
         return PP_OK;
     };
     PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[12], 0));  // the packed fields
@@ -1345,14 +1348,18 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     }
     PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     PP_CHECK(side_work());  // fork after the wave gather (measured best: beside the posts, not the gather)
-    // 5. posts
+    // 5. posts: the wave post on the side stream after the small peaks' work
+    // (PP_WAVE_ON_SIDE), beside the lane post on the main stream
+    hipStream_t wave_stream = PP_WAVE_ON_SIDE ? ctx->side : ctx->stream;
+    if (PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev[13], wave_stream));
     if (coop_end > 0) {
         const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * PP_WAVE_GRID_CU));
-        hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps, (int64_t)0, coop_end,
+        hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, wave_stream, (const PeakParam *)pps, (int64_t)0, coop_end,
                            d_rec, (const int64_t *)d_off, C, S.R);
         PP_HIP(hipGetLastError());
     }
-    PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev[PP_WAVE_ON_SIDE ? 10 : 5], wave_stream));
     if (ctx->post_width <= 1) {
         PP_CHECK(post_lane(coop_end, small_lo, false));
     } else if (small_lo > coop_end) {
@@ -1369,7 +1376,8 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipGetLastError());
     }
     PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
-    // 6. finish for the wave-gathered peaks
+    // 6. finish for the wave-gathered peaks (after the wave post's statuses)
+    if (PP_WAVE_ON_SIDE) PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[10], 0));
     if (small_lo > 0) {
         int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * PP_FINISH_GRID_CU));
         gf = (gf + nx - 1) / nx * nx;
@@ -1418,9 +1426,11 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // stage times
     PP_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
     PP_HIP(hipEventSynchronize(ctx->ev[8]));
-    // main stream: 2 start | prepare | 3 | k_gather | 4 | wave post | 5 | lane post |
-    // 6 | finish | 7 | join | 8;  side: 11 (fork) | k_small | 9 | small post | 10
-    const int from[Timing::NSTAGE] = {2, 3, 11, 4, 5, 6, 9}, to[Timing::NSTAGE] = {3, 4, 9, 5, 6, 7, 10};
+    // main stream: 2 start | prepare | 3 | k_gather | 4 | [wave post] | 5 | lane post |
+    // 6 | finish | 7 | join | 8;  side: 11 (fork) | k_small | 9 | small post | 13 |
+    // [wave post] | 10 (the wave post runs on one of the two)
+    const int from[Timing::NSTAGE] = {2, 3, 11, PP_WAVE_ON_SIDE ? 13 : 4, 5, 6, 9};
+    const int to[Timing::NSTAGE] = {3, 4, 9, PP_WAVE_ON_SIDE ? 10 : 5, 6, 7, PP_WAVE_ON_SIDE ? 13 : 10};
     for (int q = 0; q < Timing::NSTAGE; q++) {
         float t = 0.0f;
         (void)hipEventElapsedTime(&t, ctx->ev[from[q]], ctx->ev[to[q]]);
