@@ -203,7 +203,7 @@ int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
  *   [1] shells per LDS chunk of the wave gather (8..1024),
  *   [2] shell count above which a peak's Float64 post runs on a whole wave,
  *   [3] cap on resident gather waves per CU,
- *   [4] lanes per peak of the Float64 post below [2] (1, 4, 8 or 16),
+ *   [4] lanes per peak of the Float64 post below [2] (1, 2, 4, 8 or 16),
  *   [5] with one lane per peak: virialisation walks longer than this many
  *       shells are re-run one wave per peak (0: no limit).
  * set (may be NULL): new values, entries <= 0 keep the current one except

@@ -1370,7 +1370,8 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
             hipLaunchKernelGGL(kern, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, coop_end, small_lo,
                                d_rec, (const int64_t *)d_off, C, S.R);
         };
-        if (W == 4) launch(k_post_group<4>);
+        if (W == 2) launch(k_post_group<2>);
+        else if (W == 4) launch(k_post_group<4>);
         else if (W == 8) launch(k_post_group<8>);
         else launch(k_post_group<16>);
         PP_HIP(hipGetLastError());

@@ -225,7 +225,7 @@ struct Strain {
 // per wave.  All lanes of a group take the same branches (they hold the same
 // peak), so the group's shuffles and ballots see all of its lanes.
 template <int W> struct Exec {
-    static_assert(W == 1 || W == 4 || W == 8 || W == 16 || W == 32 || W == 64, "lane-group width");
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8 || W == 16 || W == 32 || W == 64, "lane-group width");
     static constexpr bool COOP = W > 1;
     __device__ static int lane() { return W == 1 ? 0 : (int)(threadIdx.x & (W - 1)); }
     __device__ static int base() { return (int)(threadIdx.x & 63) & ~(W - 1); }

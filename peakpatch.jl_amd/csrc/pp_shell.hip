@@ -670,11 +670,11 @@ int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
     static const int cand_chunk[] = {32, 64, 128, 256};
     static const int cand_coop[] = {32, 64, 128, 256, 1 << 20};
     static const int cand_waves[] = {8, 12, 16, 24, 32};
-    static const int cand_width[] = {1, 4, 8, 16};
+    static const int cand_width[] = {1, 2, 4, 8, 16};
     static const int cand_walk[] = {0, 8, 16, 32, 64};
     struct Knob { int *v; const int *c; int nc; };
     const Knob knobs[] = {{&ctx->chunk_shells, cand_chunk, 4}, {&ctx->coop_m, cand_coop, 5},
-                          {&ctx->post_width, cand_width, 4}, {&ctx->lane_walk, cand_walk, 5},
+                          {&ctx->post_width, cand_width, 5}, {&ctx->lane_walk, cand_walk, 5},
                           {&ctx->gather_waves_cu, cand_waves, 5},
                           {&ctx->tiny_m, cand_small, 4}};
     for (const Knob &k : knobs) {

@@ -167,15 +167,17 @@ TUNINGS = [
     dict(small_m=16, chunk_shells=16, coop_m=1 << 20, gather_waves_cu=32, post_width=1, lane_walk=3),
     # one chunk, wave posts only
     dict(small_m=6, chunk_shells=1024, coop_m=1, gather_waves_cu=1, post_width=1, lane_walk=0),
+    # 2-lane group posts
+    dict(small_m=4, chunk_shells=64, coop_m=1 << 20, gather_waves_cu=16, post_width=2, lane_walk=0),
     # 4-lane group posts
-    dict(small_m=4, chunk_shells=64, coop_m=1 << 20, gather_waves_cu=16, post_width=4, lane_walk=0),
+    dict(small_m=12, chunk_shells=32, coop_m=1 << 20, gather_waves_cu=8, post_width=4, lane_walk=0),
     # 16-lane group posts
     dict(small_m=8, chunk_shells=128, coop_m=40, gather_waves_cu=16, post_width=16, lane_walk=0),
 ]
 
 
-@pytest.mark.parametrize("tune", TUNINGS, ids=["gather-all", "lane-post-deferred", "wave-post", "group4-post",
-                                               "group16-post"])
+@pytest.mark.parametrize("tune", TUNINGS, ids=["gather-all", "lane-post-deferred", "wave-post", "group2-post",
+                                               "group4-post", "group16-post"])
 def test_persistent_tunables_keep_results(ctx, tune):
     """The persistent form's launch parameters move peaks between code paths
     (small waves / chunk tables / lane, lane-group or wave post); every
@@ -199,7 +201,7 @@ def test_autotune_keeps_results(ctx):
     try:
         chosen = run.autotune()
         assert chosen["chunk_shells"] in (32, 64, 128, 256) and chosen["small_m"] in (0, 8, 12, 16)
-        assert chosen["post_width"] in (1, 4, 8, 16) and chosen["lane_walk"] in (0, 8, 16, 32, 64)
+        assert chosen["post_width"] in (1, 2, 4, 8, 16) and chosen["lane_walk"] in (0, 8, 16, 32, 64)
         r, mask = run(SHELL_PERSISTENT)
     finally:
         ctx.tunables(**before)
