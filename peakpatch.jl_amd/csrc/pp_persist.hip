@@ -594,6 +594,9 @@ struct AosProfile {
 #ifndef PP_PACK_Q4
 #define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
 #endif
+#ifndef PP_POST_SORT
+#define PP_POST_SORT 1  // lane post of the wave-gathered peaks in (m, fcrit crossing) order
+#endif
 #ifndef PP_WAVE_ON_SIDE
 #define PP_WAVE_ON_SIDE 1  // the wave post runs on the side stream after the small peaks (beside the lane post)
 #endif
@@ -613,16 +616,35 @@ struct AosProfile {
 #define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
 #endif
 
+// Sort key of the lane post's order: m descending, then the fcrit crossing
+// descending, so the 64 lanes of a wave mostly start their walks at the same
+// shell and step through the same shell bands together.
+__global__ void k_crossing_keys(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
+                                const int64_t *__restrict__ prof_off, const double *__restrict__ rad,
+                                uint64_t *__restrict__ keys, int32_t *__restrict__ pos)
+{
+    for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += (int64_t)gridDim.x * blockDim.x) {
+        const PeakParam pk = pps[t];
+        const int64_t rb = prof_off[t];
+        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, rad, false};
+        const int mc = (pk.flags & 2) ? 0 : crossing_m0(P, pk);
+        keys[t - lo] = (uint64_t)(0xffff - min(pk.m, 0xffff)) << 16 | (uint64_t)(0xffff - min(mc, 0xffff));
+        pos[t - lo] = (int32_t)t;
+    }
+}
+
 // One lane per peak.  A walk longer than max_steps (a wave runs in step with
-// its slowest lane) is handed to the wave post through the defer list.
+// its slowest lane) is handed to the wave post through the defer list.  With
+// a list, lane t takes the peak at list[t - lo].
 template <bool FULL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST_LANE_WAVES)))
 k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
             const int64_t *__restrict__ prof_off, PostConst C, ResultView R, int has_lap, int max_steps,
-            int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer)
+            int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer, const int32_t *__restrict__ list)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t idx = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < hi; idx += stride) {
+    for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += stride) {
+        const int64_t idx = list ? (int64_t)list[t - lo] : t;
         const PeakParam pk = pps[idx];
         const int64_t rb = prof_off[idx];
         AosProfile<FULL> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, has_lap != 0};
@@ -1282,12 +1304,27 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         if (full) {
             hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->side, (const PeakParam *)pps, lo, hi,
                                d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0, 0x7fffffff, d_defer,
-                               d_ndefer);
+                               d_ndefer, (const int32_t *)nullptr);
         } else {
             const int budget = ctx->lane_walk > 0 ? ctx->lane_walk : 0x7fffffff;
             hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, (int *)d_ndefer, (int64_t)1, 0);
+            // lanes of a wave start at the same crossing where possible
+            const int32_t *list = nullptr;
+            if (PP_POST_SORT && hi - lo > 64) {
+                const int64_t nl = hi - lo;
+                hipLaunchKernelGGL(k_crossing_keys, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream,
+                                   (const PeakParam *)pps, lo, hi, d_rec, (const int64_t *)d_off, C.rad, W.ka, W.iota);
+                PP_HIP(hipGetLastError());
+                size_t tmp = 0;
+                PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, W.ka, W.kb, W.iota, W.mo, (int)nl, 0, 32,
+                                                          ctx->stream));
+                PP_CHECK(ctx->d_cub.grow(tmp));
+                PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, W.ka, W.kb, W.iota, W.mo, (int)nl, 0,
+                                                          32, ctx->stream));
+                list = W.mo;
+            }
             hipLaunchKernelGGL(k_post_lane<false>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
-                               d_rec, (const int64_t *)d_off, C, S.R, 0, budget, d_defer, d_ndefer);
+                               d_rec, (const int64_t *)d_off, C, S.R, 0, budget, d_defer, d_ndefer, list);
             if (ctx->lane_walk > 0) {
                 const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(hi - lo, (int64_t)ctx->num_cu * 16));
                 hipLaunchKernelGGL(k_post_defer, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps,

@@ -417,6 +417,28 @@ __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, in
     }
 }
 
+// The fcrit crossing of one peak, serially (RadialShell.jl:492-538): the m0
+// the inward walk starts from, as analyse_post<1> finds it; 0 when there is
+// none (a DUMP_NO_FCRIT peak).  Used to order the lane post's peaks.
+template <class PV>
+__device__ int crossing_m0(const PV &P, const PeakParam &pk)
+{
+    const double fcrit = pk.fcrit;
+    const int m = pk.m;
+    int m0 = pk.m0;
+    if (P.Fbar(m0) >= fcrit) {
+        for (int mm = m0; mm <= m; mm++)
+            if (P.Fbar(mm) < fcrit) return mm;
+        return m0;
+    }
+    const int mstart = m0 - 1 > 1 ? m0 - 1 : 1;
+    for (int mp = mstart; mp >= 1; mp--) {
+        if (P.Fbar(mp) >= fcrit) return m0;
+        m0 = mp;
+    }
+    return 0;
+}
+
 // The per-peak analysis after the gather.  P provides rad(s), Fbar(s),
 // count(s) and strain_inputs(s); with FINISH it also provides S / S2 / lap
 // sums per shell and computes Sbar, Sbar2, d2F and the mask here; without
