@@ -199,7 +199,8 @@ int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
 
 /* Launch parameters of the persistent form, in this order:
  *   [0] small-peak threshold in finalised shells (peaks with m at or below it
- *       are packed several per wave; 0 disables that path; at most 16),
+ *       are gathered cell by cell, floor(64 / m) per wave; 0 disables that
+ *       path; at most 64),
  *   [1] shells per LDS chunk of the wave gather (8..1024),
  *   [2] shell count above which a peak's Float64 post runs on a whole wave,
  *   [3] cap on resident gather waves per CU,

@@ -279,7 +279,7 @@ int pp_shell_tunables(pp_ctx *ctx, const int32_t *set, int32_t *get)
 {
     PP_ARG(ctx, "null context");
     if (set) {
-        if (set[0] > 16) { set_error("small-peak threshold above 16 shells"); return PP_ERR_ARG; }
+        if (set[0] > 64) { set_error("small-peak threshold above 64 shells"); return PP_ERR_ARG; }
         if (set[1] > 0 && (set[1] < 8 || set[1] > 1024)) { set_error("chunk shells outside [8, 1024]"); return PP_ERR_ARG; }
         if (set[3] > 64) { set_error("more than 64 gather waves per CU"); return PP_ERR_ARG; }
         if (set[4] > 0 && set[4] != 1 && set[4] != 2 && set[4] != 4 && set[4] != 8 && set[4] != 16) {

@@ -60,7 +60,7 @@ enum : int {
     P_G = 0, P_GF = 3, P_E = 6, P_F = 12, P_N = 13, P_NB = 14 /* fields binned by k_gather */,
     P_FB = 14 /* Fbar: double in slots 14-15 */, P_S = 16, P_S2 = 19, P_L = 22
 };
-constexpr int SMALL_MAX = 16;  // largest m whose groups fit one 64-lane batch (62 groups)
+constexpr int SMALL_MAX = 64;  // largest m of the cell-based small-peak gather (its P m bins fit 64 lanes)
 // Work placement.  The gathers read the fields around each peak; peaks of
 // equal m are taken in Morton order of their centres so that the waves in
 // flight share cache lines, and each XCD (its own L2) works through its own

@@ -666,7 +666,7 @@ int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
     };
     double t = 0;
     PP_CHECK(timed(t));  // warm-up: tables, buffers
-    static const int cand_small[] = {0, 8, 12, 16};
+    static const int cand_small[] = {0, 8, 12, 16, 24, 32, 48, 64};
     static const int cand_chunk[] = {32, 64, 128, 256};
     static const int cand_coop[] = {32, 64, 128, 256, 1 << 20};
     static const int cand_waves[] = {8, 12, 16, 24, 32};
@@ -676,7 +676,7 @@ int pp_autotune_shell(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch 
     const Knob knobs[] = {{&ctx->chunk_shells, cand_chunk, 4}, {&ctx->coop_m, cand_coop, 5},
                           {&ctx->post_width, cand_width, 5}, {&ctx->lane_walk, cand_walk, 5},
                           {&ctx->gather_waves_cu, cand_waves, 5},
-                          {&ctx->tiny_m, cand_small, 4}};
+                          {&ctx->tiny_m, cand_small, 8}};
     for (const Knob &k : knobs) {
         int bestv = *k.v;
         double bt = 1e300;

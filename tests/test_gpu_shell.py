@@ -169,8 +169,8 @@ TUNINGS = [
     dict(small_m=6, chunk_shells=1024, coop_m=1, gather_waves_cu=1, post_width=1, lane_walk=0),
     # 2-lane group posts
     dict(small_m=4, chunk_shells=64, coop_m=1 << 20, gather_waves_cu=16, post_width=2, lane_walk=0),
-    # 4-lane group posts
-    dict(small_m=12, chunk_shells=32, coop_m=1 << 20, gather_waves_cu=8, post_width=4, lane_walk=0),
+    # 4-lane group posts, the cell-based small gather up to 64 shells
+    dict(small_m=64, chunk_shells=32, coop_m=1 << 20, gather_waves_cu=8, post_width=4, lane_walk=0),
     # 16-lane group posts
     dict(small_m=8, chunk_shells=128, coop_m=40, gather_waves_cu=16, post_width=16, lane_walk=0),
 ]
@@ -200,7 +200,7 @@ def test_autotune_keeps_results(ctx):
     before = ctx.tunables()
     try:
         chosen = run.autotune()
-        assert chosen["chunk_shells"] in (32, 64, 128, 256) and chosen["small_m"] in (0, 8, 12, 16)
+        assert chosen["chunk_shells"] in (32, 64, 128, 256) and chosen["small_m"] in (0, 8, 12, 16, 24, 32, 48, 64)
         assert chosen["post_width"] in (1, 2, 4, 8, 16) and chosen["lane_walk"] in (0, 8, 16, 32, 64)
         r, mask = run(SHELL_PERSISTENT)
     finally:
