@@ -616,21 +616,27 @@ struct AosProfile {
 #define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
 #endif
 
-// Sort key of the lane post's order: m descending, then the fcrit crossing
-// descending, so the 64 lanes of a wave mostly start their walks at the same
-// shell and step through the same shell bands together.
+// The lane post's order: m descending, then the fcrit crossing descending,
+// ties in position order (a stable radix sort of these keys), so the 64
+// lanes of a wave mostly start their walks at the same shell and step
+// through the same shell bands together.
 __global__ void k_crossing_keys(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
                                 const int64_t *__restrict__ prof_off, const double *__restrict__ rad,
-                                uint64_t *__restrict__ keys, int32_t *__restrict__ pos)
+                                uint32_t *__restrict__ keys)
 {
     for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += (int64_t)gridDim.x * blockDim.x) {
         const PeakParam pk = pps[t];
         const int64_t rb = prof_off[t];
         AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, rad, false};
         const int mc = (pk.flags & 2) ? 0 : crossing_m0(P, pk);
-        keys[t - lo] = (uint64_t)(0xffff - min(pk.m, 0xffff)) << 16 | (uint64_t)(0xffff - min(mc, 0xffff));
-        pos[t - lo] = (int32_t)t;
+        keys[t - lo] = (uint32_t)(0xffff - min(pk.m, 0xffff)) << 16 | (uint32_t)(0xffff - min(max(mc, 0), 0xffff));
     }
+}
+
+__global__ void k_iota_from(int32_t *__restrict__ o, int64_t n, int32_t first)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        o[p] = first + (int32_t)p;
 }
 
 // One lane per peak.  A walk longer than max_steps (a wave runs in step with
@@ -1297,6 +1303,28 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_CHECK(ctx->d_k2.grow(256 + (size_t)std::max<int64_t>(small_lo - coop_end, 1) * sizeof(int64_t)));
     unsigned int *d_ndefer = ctx->d_k2.as<unsigned int>();
     int64_t *d_defer = (int64_t *)(ctx->d_k2.as<char>() + 256);
+    // The lane post's order of [coop_end, small_lo) by (m, fcrit crossing), built
+    // on the main stream before the side stream's kernels start (beside them
+    // its small grids wait for free CUs)
+    const int32_t *cross_list = nullptr;
+    const int64_t cross_lo = coop_end, cross_hi = small_lo;
+    auto cross_order = [&]() -> int {
+        const int64_t nl = cross_hi - cross_lo;
+        if (!PP_POST_SORT || ctx->post_width > 1 || nl <= 64) return PP_OK;
+        uint32_t *ka = (uint32_t *)W.ka, *kb = (uint32_t *)W.kb;
+        hipLaunchKernelGGL(k_crossing_keys, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, (const PeakParam *)pps,
+                           cross_lo, cross_hi, d_rec, (const int64_t *)d_off, C.rad, ka);
+        hipLaunchKernelGGL(k_iota_from, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, W.iota, nl, (int32_t)cross_lo);
+        PP_HIP(hipGetLastError());
+        // stable: equal keys keep the caller's order (neighbouring profiles and result rows)
+        size_t tmp = 0;
+        PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 32, ctx->stream));
+        PP_CHECK(ctx->d_cub.grow(tmp));
+        PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 32,
+                                                  ctx->stream));
+        cross_list = W.mo;
+        return PP_OK;
+    };
     auto post_lane = [&](int64_t lo, int64_t hi, bool full) -> int {
         if (hi <= lo) return PP_OK;
         const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64,
@@ -1308,21 +1336,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         } else {
             const int budget = ctx->lane_walk > 0 ? ctx->lane_walk : 0x7fffffff;
             hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, (int *)d_ndefer, (int64_t)1, 0);
-            // lanes of a wave start at the same crossing where possible
-            const int32_t *list = nullptr;
-            if (PP_POST_SORT && hi - lo > 64) {
-                const int64_t nl = hi - lo;
-                hipLaunchKernelGGL(k_crossing_keys, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream,
-                                   (const PeakParam *)pps, lo, hi, d_rec, (const int64_t *)d_off, C.rad, W.ka, W.iota);
-                PP_HIP(hipGetLastError());
-                size_t tmp = 0;
-                PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, W.ka, W.kb, W.iota, W.mo, (int)nl, 0, 32,
-                                                          ctx->stream));
-                PP_CHECK(ctx->d_cub.grow(tmp));
-                PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, W.ka, W.kb, W.iota, W.mo, (int)nl, 0,
-                                                          32, ctx->stream));
-                list = W.mo;
-            }
+            // lanes of a wave start at the same crossing where possible (the
+            // order is built before the fork, see cross_order)
+            const int32_t *list = (lo == cross_lo && hi == cross_hi) ? cross_list : nullptr;
             hipLaunchKernelGGL(k_post_lane<false>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
                                d_rec, (const int64_t *)d_off, C, S.R, 0, budget, d_defer, d_ndefer, list);
             if (ctx->lane_walk > 0) {
@@ -1384,6 +1400,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipGetLastError());
     }
     PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    PP_CHECK(cross_order());
     PP_CHECK(side_work());  // fork after the wave gather (measured best: beside the posts, not the gather)
     // 5. posts: the wave post on the side stream after the small peaks' work
     // (PP_WAVE_ON_SIDE), beside the lane post on the main stream
