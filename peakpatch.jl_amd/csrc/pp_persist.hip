@@ -69,6 +69,9 @@ constexpr int NXCD = 8;
 #ifndef PP_LOCALITY
 #define PP_LOCALITY 2  // 0: one list in m order, 1: per-XCD lists, 2: + Morton order in the lists
 #endif
+#ifndef PP_GATHER_CLASS
+#define PP_GATHER_CLASS 1  // Morton order inside size classes floor(log2 m) rather than inside each m
+#endif
 
 // (delta, delta transposed, eta x, eta y) of every cell as one float4, so the
 // gathers read the four with one 16-byte load (and a 4-byte one for eta z):
@@ -162,8 +165,19 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v)
     return x;
 }
 
-// sort key: m descending, then the Morton code of the centre cell (nbits per
-// axis), so peaks of equal m that are processed together lie close in space
+// Size class of the wave gather's order: floor(log2 m) (PP_GATHER_CLASS), or
+// m itself.  Peaks of one class take the same order of magnitude of work;
+// the fields around a place are read for all its peaks of a class together.
+__host__ __device__ __forceinline__ int gather_class(int m)
+{
+    if (!PP_GATHER_CLASS) return m;
+    int c = 0;
+    while ((2 << c) <= m) c++;
+    return c;
+}
+
+// sort key: size class descending, then the Morton code of the centre cell
+// (nbits per axis), so peaks processed together lie close in space
 __global__ void k_keys_morton(const PeakParam *__restrict__ pp, int64_t np, int nbits, uint64_t *__restrict__ keys)
 {
     const uint32_t msk = (1u << nbits) - 1u;
@@ -171,19 +185,22 @@ __global__ void k_keys_morton(const PeakParam *__restrict__ pp, int64_t np, int 
         const PeakParam q = pp[p];
         const uint64_t mort = spread3((uint32_t)q.ci & msk) | spread3((uint32_t)q.cj & msk) << 1 |
                               spread3((uint32_t)q.ck & msk) << 2;
-        keys[p] = (uint64_t)(0xffff - min(q.m, 0xffff)) << (3 * nbits) | mort;
+        keys[p] = (uint64_t)(0xffff - min(gather_class(q.m), 0xffff)) << (3 * nbits) | mort;
     }
 }
 
-// Per-XCD work lists of sorted positions [0, hi): every run of equal m
-// (tab row m: first position, length, then the list offset of each of its
-// nx parts) is cut into nx parts of consecutive entries of src (the
-// positions themselves if src is null); part x is appended to list x.
+// Per-XCD work lists of the entries [0, hi) of src (positions of the ordered
+// peaks; the positions themselves if src is null), which runs of equal group
+// g follow each other in (g = m, or the size class with by_class): tab row g
+// holds the run's first entry, its length, then the list offset of each of
+// its nx parts; the run is cut into nx parts of consecutive entries and part
+// x is appended to list x.
 __global__ void k_xcd_order(const int32_t *__restrict__ m_sorted, const int32_t *__restrict__ src, int64_t hi,
-                            const int64_t *__restrict__ tab, int nx, int32_t *__restrict__ xorder)
+                            const int64_t *__restrict__ tab, int nx, int by_class, int32_t *__restrict__ xorder)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < hi; p += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t *t = tab + (int64_t)m_sorted[p] * (2 + NXCD);
+        const int m = m_sorted[src ? src[p] : p];
+        const int64_t *t = tab + (int64_t)(by_class ? gather_class(m) : m) * (2 + NXCD);
         const int64_t r = p - t[0], len = t[1];
         int x = (int)min<int64_t>(nx - 1, r * nx / len);
         while (x + 1 < nx && len * (x + 1) / nx <= r) x++;
@@ -854,7 +871,8 @@ struct PersistState {
     int gsteps = 0;                                   // shuffle-tree steps of the gather tables
     std::vector<SmallSeg> segs_host;
     std::vector<SmallPart> parts_host;                // per-XCD block lists of k_small
-    std::vector<int64_t> xtab_host;                   // per m: k_xcd_order's table row
+    std::vector<int64_t> xtab_host;                   // per m: k_xcd_order's table row (finish lists)
+    std::vector<int64_t> xtab_g_host;                 // per group: the same for the gather lists
     DevBuf d_desc, d_full_off, d_full_nb, d_part_off, d_part_nb, d_scells, d_seg, d_xtab;
     bool dirty = true;
 };
@@ -1209,46 +1227,63 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_CHECK(ctx->d_misc.grow(8192));
     int *d_q = (int *)(ctx->d_misc.as<char>() + 4096);  // NXCD queue counters, 256 B apart
     // per-XCD lists of the wave-gathered peaks [0, small_lo): part x of every
-    // run of equal m goes to list x
+    // run of one group goes to list x; the finish pass's groups are the m
+    // (its list keeps the caller's order), the gather's the size classes in
+    // Morton order
     const int nx = PP_LOCALITY >= 1 ? NXCD : 1;
-    int64_t xb[NXCD + 1];
-    {
-        st.xtab_host.assign((size_t)nb * (2 + NXCD), 0);
+    int64_t xb[NXCD + 1], xbg[NXCD + 1];
+    // tab rows for groups ng (group of m: grp(m)), offsets into out; fills xbo
+    auto build_xtab = [&](int ng, auto grp, std::vector<int64_t> &tab, int64_t *xbo) -> int {
+        tab.assign((size_t)ng * (2 + NXCD), 0);
+        std::vector<int64_t> len(ng, 0), first(ng, 0);
+        for (int m = nb - 1; m > small_m; m--) len[grp(m)] += hist[m];
+        {   // groups follow each other with decreasing group index (m descending)
+            int64_t pos = 0;
+            for (int g = ng - 1; g >= 0; g--) { first[g] = pos; pos += len[g]; }
+        }
         int64_t cur[NXCD] = {0};
-        for (int m = nb - 1; m > small_m; m--)
-            for (int x = 0; x < nx; x++) cur[x] += (int64_t)hist[m] * (x + 1) / nx - (int64_t)hist[m] * x / nx;
-        xb[0] = 0;
-        for (int x = 0; x < NXCD; x++) xb[x + 1] = xb[x] + (x < nx ? cur[x] : 0);
-        for (int x = 0; x < nx; x++) cur[x] = xb[x];
-        for (int m = nb - 1; m > small_m; m--) {
-            const int64_t len = hist[m];
-            if (!len) continue;
-            int64_t *t = &st.xtab_host[(size_t)m * (2 + NXCD)];
-            t[0] = cnt_gt(m);
-            t[1] = len;
+        for (int g = ng - 1; g >= 0; g--)
+            for (int x = 0; x < nx; x++) cur[x] += len[g] * (x + 1) / nx - len[g] * x / nx;
+        xbo[0] = 0;
+        for (int x = 0; x < NXCD; x++) xbo[x + 1] = xbo[x] + (x < nx ? cur[x] : 0);
+        for (int x = 0; x < nx; x++) cur[x] = xbo[x];
+        for (int g = ng - 1; g >= 0; g--) {
+            if (!len[g]) continue;
+            int64_t *t = &tab[(size_t)g * (2 + NXCD)];
+            t[0] = first[g];
+            t[1] = len[g];
             for (int x = 0; x < nx; x++) {
                 t[2 + x] = cur[x];
-                cur[x] += len * (x + 1) / nx - len * x / nx;
+                cur[x] += len[g] * (x + 1) / nx - len[g] * x / nx;
             }
         }
-        if (xb[NXCD] != small_lo) {
-            set_error("per-XCD lists hold %lld of %lld peaks", (long long)xb[NXCD], (long long)small_lo);
+        if (xbo[NXCD] != small_lo) {
+            set_error("per-XCD lists hold %lld of %lld peaks", (long long)xbo[NXCD], (long long)small_lo);
             return PP_ERR_STATE;
         }
+        return PP_OK;
+    };
+    {
+        PP_CHECK(build_xtab(nb, [](int m) { return m; }, st.xtab_host, xb));
+        const bool by_class = PP_LOCALITY >= 2 && PP_GATHER_CLASS;
+        PP_CHECK(build_xtab(by_class ? 32 : nb, [&](int m) { return by_class ? gather_class(m) : m; }, st.xtab_g_host,
+                            xbg));
         if (small_lo > 0) {
-            PP_CHECK(st.d_xtab.grow(st.xtab_host.size() * sizeof(int64_t)));
-            PP_HIP(hipMemcpyAsync(st.d_xtab.ptr, st.xtab_host.data(), st.xtab_host.size() * sizeof(int64_t),
-                                  hipMemcpyHostToDevice, ctx->stream));
+            const size_t tb = st.xtab_host.size() * sizeof(int64_t), tg = st.xtab_g_host.size() * sizeof(int64_t);
+            PP_CHECK(st.d_xtab.grow(tb + tg));
+            PP_HIP(hipMemcpyAsync(st.d_xtab.ptr, st.xtab_host.data(), tb, hipMemcpyHostToDevice, ctx->stream));
+            PP_HIP(hipMemcpyAsync(st.d_xtab.as<char>() + tb, st.xtab_g_host.data(), tg, hipMemcpyHostToDevice,
+                                  ctx->stream));
             // finish: parts of consecutive positions; gather: of the Morton order
             hipLaunchKernelGGL(k_xcd_order, dim3(grid1d(ctx, small_lo)), dim3(256), 0, ctx->stream,
                                (const int32_t *)m_sorted, (const int32_t *)nullptr, small_lo,
-                               st.d_xtab.as<int64_t>(), nx, W.xf);
+                               st.d_xtab.as<int64_t>(), nx, 0, W.xf);
             PP_HIP(hipGetLastError());
             if (PP_LOCALITY >= 2) {
                 PP_CHECK(morton_order(ctx, pps, small_lo, n, W));
                 hipLaunchKernelGGL(k_xcd_order, dim3(grid1d(ctx, small_lo)), dim3(256), 0, ctx->stream,
                                    (const int32_t *)m_sorted, (const int32_t *)W.mo, small_lo,
-                                   st.d_xtab.as<int64_t>(), nx, W.xg);
+                                   (const int64_t *)(st.d_xtab.as<char>() + tb), nx, by_class ? 1 : 0, W.xg);
                 PP_HIP(hipGetLastError());
             } else {
                 W.xg = W.xf;
@@ -1393,7 +1428,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         A.part_off = st.d_part_off.as<int64_t>(); A.part_nb = st.d_part_nb.as<int32_t>();
         A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad; A.queue = d_q; A.cs = cs; A.steps = st.gsteps;
         A.xorder = W.xg; A.nx = nx;
-        for (int x = 0; x <= NXCD; x++) A.xb[x] = xb[x];
+        for (int x = 0; x <= NXCD; x++) A.xb[x] = xbg[x];
         if (lds > 64 * 1024)
             (void)hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_gather, dim3(grid), dim3(64), lds, ctx->stream, A);
