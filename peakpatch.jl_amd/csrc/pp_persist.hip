@@ -70,7 +70,7 @@ constexpr int NXCD = 8;
 #define PP_LOCALITY 2  // 0: one list in m order, 1: per-XCD lists, 2: + Morton order in the lists
 #endif
 #ifndef PP_GATHER_CLASS
-#define PP_GATHER_CLASS 1  // Morton order inside size classes floor(log4 m) rather than inside each m
+#define PP_GATHER_CLASS 1  // Morton order inside size classes floor(log16 m) rather than inside each m
 #endif
 
 // (delta, delta transposed, eta x, eta y) of every cell as one float4, so the
@@ -165,17 +165,17 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v)
     return x;
 }
 
-// Size class of the wave gather's order: floor(log4 m) (PP_GATHER_CLASS), or
-// m itself.  Peaks of one class take the same order of magnitude of work
-// (the largest class still starts first); the fields around a place are read
-// for all its peaks of a class together.  Measured at c3: per m 6.6 ms,
-// log2 classes 6.0, log4 5.6, one class 6.4 (its largest peaks end last).
+// Size class of the wave gather's order: floor(log16 m) (PP_GATHER_CLASS), or
+// m itself.  The largest class still starts first; the fields around a place
+// are read for all its peaks of a class together.  Measured at c3 (classes
+// m < 256 and m >= 256 there): per m 6.6 ms, log2 classes 6.0, log4 5.6,
+// log16 5.4, one class 6.4 (its largest peaks end last), m >= 1024 first 6.5.
 __host__ __device__ __forceinline__ int gather_class(int m)
 {
     if (!PP_GATHER_CLASS) return m;
     int c = 0;
     while ((2 << c) <= m) c++;
-    return c / 2;
+    return c / 4;
 }
 
 // sort key: size class descending, then the Morton code of the centre cell
