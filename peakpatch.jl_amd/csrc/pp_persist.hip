@@ -44,14 +44,16 @@
 namespace pp {
 
 constexpr int PR = 24;  // floats per shell record of the AoS profile
-// In memory the profile is two arrays of 12-float shell records: the strain
-// inputs G(3) Gf(3) E(6), which the posts' kernel_strain reads for every shell
-// of a band, and the rest (F, count, Fbar, S, S2, lapd), so the band loops
-// move 48 bytes per shell.  In registers a record is v[PR]: the strain part
-// in v[0..11], the rest in v[12..23].
+// In memory the profile is three arrays of shell records: the strain inputs
+// E(6), which every step of the posts' inward walk reads for each shell of its
+// band, the gradient inputs G(3) Gf(3), read at the few steps whose gradients
+// are kept, and the rest (F, count, Fbar, S, S2, lapd), so a walk step moves
+// 24 bytes per shell and its cache lines hold nothing else.  In registers a
+// record is v[PR]: the strain part in v[0..11], the rest in v[12..23].
 constexpr int PSR = 12;
+constexpr int PSE = 6;  // floats per record of the E and the G arrays
 struct ProfBuf {
-    float *s, *a;  // record r: strain part at s + 12 r, the rest at a + 12 r
+    float *e, *g, *a;  // record r: E at e + 6 r, G Gf at g + 6 r, the rest at a + 12 r
 };
 // record: G(3) Gf(3) E(6) | F, count | Fbar (double) | S(3) S2(3) lapd | pad;
 // the 12 strain inputs load as three 16-byte vectors.  S, S2, lapd are filled
@@ -257,9 +259,13 @@ __device__ __forceinline__ void group_sum(const GatherFields &F, const PeakParam
 // Record r: the strain part and the first naux float4 of the rest
 __device__ __forceinline__ void store_record(ProfBuf p, int64_t r, const float v[PR], int naux)
 {
-    float4 *os = (float4 *)(p.s + r * PSR), *oa = (float4 *)(p.a + r * PSR);
+    float2 *og = (float2 *)(p.g + r * PSE), *oe = (float2 *)(p.e + r * PSE);
+    float4 *oa = (float4 *)(p.a + r * PSR);
 #pragma unroll
-    for (int q = 0; q < 3; q++) os[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    for (int q = 0; q < 3; q++) {
+        og[q] = make_float2(v[P_G + 2 * q], v[P_G + 2 * q + 1]);
+        oe[q] = make_float2(v[P_E + 2 * q], v[P_E + 2 * q + 1]);
+    }
     for (int q = 0; q < naux; q++)
         oa[q] = make_float4(v[PSR + 4 * q], v[PSR + 4 * q + 1], v[PSR + 4 * q + 2], v[PSR + 4 * q + 3]);
 }
@@ -588,10 +594,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherArgs A)
 // --------------------------------------------------------------- post
 template <bool FULL>
 struct AosProfile {
-    const float *sr, *ax;  // this peak's records (strain part, the rest)
+    const float *se, *sg, *ax;  // this peak's records (E, G Gf, the rest)
     const double *radt;
     bool lapd;
-    __device__ const float *r(int s) const { return sr + (int64_t)(s - 1) * PSR; }
     __device__ const float *x(int s) const { return ax + (int64_t)(s - 1) * PSR; }
     __device__ double rad(int s) const { return radt[s - 1]; }
     __device__ double Fbar(int s) const { return *(const double *)(x(s) + (P_FB - PSR)); }
@@ -600,15 +605,26 @@ struct AosProfile {
     __device__ double S2(int s, int q) const { return FULL ? (double)x(s)[P_S2 - PSR + q] : 0.0; }
     __device__ double lap(int s) const { return FULL ? (double)x(s)[P_L - PSR] : 0.0; }
     __device__ bool has_lapd() const { return FULL && lapd; }
-    __device__ void strain_raw(int s, float G[3], float Gf[3], float Es[6]) const
+    __device__ void strain_E(int s, float Es[6]) const
     {
-        const float4 *q = (const float4 *)r(s);
-        const float4 a = q[0], b = q[1], c = q[2];
-        G[0] = a.x; G[1] = a.y; G[2] = a.z;
-        Gf[0] = a.w; Gf[1] = b.x; Gf[2] = b.y;
-        Es[0] = b.z; Es[1] = b.w; Es[2] = c.x; Es[3] = c.y; Es[4] = c.z; Es[5] = c.w;
+        const float2 *q = (const float2 *)(se + (int64_t)(s - 1) * PSE);
+        const float2 a = q[0], b = q[1], c = q[2];
+        Es[0] = a.x; Es[1] = a.y; Es[2] = b.x; Es[3] = b.y; Es[4] = c.x; Es[5] = c.y;
+    }
+    __device__ void strain_G(int s, float G[3], float Gf[3]) const
+    {
+        const float2 *q = (const float2 *)(sg + (int64_t)(s - 1) * PSE);
+        const float2 a = q[0], b = q[1], c = q[2];
+        G[0] = a.x; G[1] = a.y; G[2] = b.x;
+        Gf[0] = b.y; Gf[1] = c.x; Gf[2] = c.y;
     }
 };
+
+template <bool FULL>
+__device__ __forceinline__ AosProfile<FULL> profile_at(ProfBuf p, int64_t rb, const double *rad, bool lapd)
+{
+    return AosProfile<FULL>{p.e + rb * PSE, p.g + rb * PSE, p.a + rb * PSR, rad, lapd};
+}
 
 #ifndef PP_PACK_Q4
 #define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
@@ -646,7 +662,7 @@ __global__ void k_crossing_keys(const PeakParam *__restrict__ pps, int64_t lo, i
     for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += (int64_t)gridDim.x * blockDim.x) {
         const PeakParam pk = pps[t];
         const int64_t rb = prof_off[t];
-        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, rad, false};
+        const AosProfile<false> P = profile_at<false>(prof, rb, rad, false);
         const int mc = (pk.flags & 2) ? 0 : crossing_m0(P, pk);
         keys[t - lo] = (uint32_t)(0xffff - min(pk.m, 0xffff)) << 16 | (uint32_t)(0xffff - min(max(mc, 0), 0xffff));
     }
@@ -672,7 +688,7 @@ k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf p
         const int64_t idx = list ? (int64_t)list[t - lo] : t;
         const PeakParam pk = pps[idx];
         const int64_t rb = prof_off[idx];
-        AosProfile<FULL> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, has_lap != 0};
+        const AosProfile<FULL> P = profile_at<FULL>(prof, rb, C.rad, has_lap != 0);
         if (analyse_post<1, FULL>(P, pk, C, R, pk.order, max_steps) == POST_DEFER) {
             defer[atomicAdd(ndefer, 1u)] = idx;
             continue;
@@ -691,7 +707,7 @@ __global__ void __launch_bounds__(64) k_post_defer(const PeakParam *__restrict__
         const int64_t idx = defer[t];
         const PeakParam pk = pps[idx];
         const int64_t rb = prof_off[idx];
-        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, false};
+        const AosProfile<false> P = profile_at<false>(prof, rb, C.rad, false);
         analyse_post<64, false>(P, pk, C, R, pk.order);
         if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
     }
@@ -704,7 +720,7 @@ __global__ void __launch_bounds__(64) k_post_wave(const PeakParam *__restrict__ 
     for (int64_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
         const PeakParam pk = pps[idx];
         const int64_t rb = prof_off[idx];
-        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, false};
+        const AosProfile<false> P = profile_at<false>(prof, rb, C.rad, false);
         analyse_post<64, false>(P, pk, C, R, pk.order);
         if (threadIdx.x == 0) R.nshells[pk.order] = pk.m;
     }
@@ -723,7 +739,7 @@ __global__ void __launch_bounds__(64) k_post_group(const PeakParam *__restrict__
         if (idx >= hi) continue;  // whole lane groups drop out
         const PeakParam pk = pps[idx];
         const int64_t rb = prof_off[idx];
-        AosProfile<false> P{prof.s + rb * PSR, prof.a + rb * PSR, C.rad, false};
+        const AosProfile<false> P = profile_at<false>(prof, rb, C.rad, false);
         analyse_post<W, false>(P, pk, C, R, pk.order);
         if ((threadIdx.x & (W - 1)) == 0) R.nshells[pk.order] = pk.m;
     }
@@ -1225,7 +1241,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         tot = last[0] + last[1];
     }
     PP_CHECK(ctx->d_res.grow((size_t)std::max<int64_t>(tot, 1) * PR * sizeof(float) + 4096));
-    const ProfBuf d_rec{ctx->d_res.as<float>(), ctx->d_res.as<float>() + (size_t)std::max<int64_t>(tot, 1) * PSR};
+    const size_t nrec = (size_t)std::max<int64_t>(tot, 1);
+    const ProfBuf d_rec{ctx->d_res.as<float>(), ctx->d_res.as<float>() + nrec * PSE,
+                        ctx->d_res.as<float>() + nrec * 2 * PSE};
     PP_CHECK(ctx->d_misc.grow(8192));
     int *d_q = (int *)(ctx->d_misc.as<char>() + 4096);  // NXCD queue counters, 256 B apart
     // per-XCD lists of the wave-gathered peaks [0, small_lo): part x of every

@@ -249,8 +249,12 @@ template <int W> struct Exec {
 // shell mp depends on the two radii only, so it comes from a table built once
 // per shell table (k_wtab); the row of mp holds exactly the shells with
 // (rad(mp) - rad(m1))^2 < 4, the reference's cut, so clamping [mlow, mupp]
-// to the row is the reference's skip.
-template <int W, class PV>
+// to the row is the reference's skip.  KS selects the sums formed (and the
+// profile inputs read): KS_E the strain Ebar, KS_G the gradients g and gf;
+// a sum not selected is returned as zero.  Each sum is formed in the same
+// order whatever else is selected, so its bits do not depend on KS.
+enum : int { KS_E = 1, KS_G = 2 };
+template <int W, int KS, class PV>
 __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlow, int mupp)
 {
     using X = Exec<W>;
@@ -260,26 +264,32 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
 #pragma unroll PP_STRAIN_UNROLL
     for (int m1 = a + X::lane(); m1 <= b; m1 += W) {
         const double w = C.wtab[row.x + (m1 - row.y)];
-        float G[3], Gf[3], Es[6];
-        P.strain_raw(m1, G, Gf, Es);
+        if constexpr ((KS & KS_G) != 0) {
+            float G[3], Gf[3];
+            P.strain_G(m1, G, Gf);
 #pragma unroll
-        for (int L = 0; L < 3; L++) {
-            g[L] += w * (double)G[L];
-            gf[L] += w * (double)Gf[L];
+            for (int L = 0; L < 3; L++) {
+                g[L] += w * (double)G[L];
+                gf[L] += w * (double)Gf[L];
+            }
         }
-        // Es = {SR00+SR00, SR11+SR11, SR22+SR22, SR01+SR10, SR02+SR20, SR12+SR21}
-        const double hw = 0.5 * w;
+        if constexpr ((KS & KS_E) != 0) {
+            // Es = {SR00+SR00, SR11+SR11, SR22+SR22, SR01+SR10, SR02+SR20, SR12+SR21}
+            float Es[6];
+            P.strain_E(m1, Es);
+            const double hw = 0.5 * w;
 #pragma unroll
-        for (int q = 0; q < 6; q++) E[q] -= hw * (double)Es[q];
+            for (int q = 0; q < 6; q++) E[q] -= hw * (double)Es[q];
+        }
     }
     Strain s;
     const double scale = mp > 1 ? C.wRnor / P.rad(mp) : C.aRnor;
 #pragma unroll
-    for (int i = 0; i < 6; i++) s.E[i] = X::sum(E[i]) * scale;
+    for (int i = 0; i < 6; i++) s.E[i] = (KS & KS_E) ? X::sum(E[i]) * scale : 0.0;
 #pragma unroll
     for (int L = 0; L < 3; L++) {
-        s.g[L] = X::sum(g[L]) * scale;
-        s.gf[L] = X::sum(gf[L]) * scale;
+        s.g[L] = (KS & KS_G) ? X::sum(g[L]) * scale : 0.0;
+        s.gf[L] = (KS & KS_G) ? X::sum(gf[L]) * scale : 0.0;
     }
     return s;
 }
@@ -463,7 +473,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         if (mrf > 1) { int a = first_rad_gt(C, 1, mrf - 1, rlow_rf); if (a <= mrf - 1) mlow_rf = a; }
         int mupp_rf = mrf;
         { int b = last_rad_lt(C, mrf + 1, m, P.rad(mrf) + 2.0); if (b >= mrf + 1) mupp_rf = b; }
-        Strain srf = kernel_strain<W>(P, C, mrf, mlow_rf, mupp_rf);
+        Strain srf = kernel_strain<W, KS_G>(P, C, mrf, mlow_rf, mupp_rf);
         const double gradpkrf[3] = {srf.g[0], srf.g[1], srf.g[2]};
 
         // ---- Phase 4: fcrit crossing (RadialShell.jl:492-538) ----
@@ -513,20 +523,20 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         int mlow = m0;
         if (m0 > 1) { int a = first_rad_gt(C, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
         const double ZZon = pk.ZZon;
-        Strain s0 = kernel_strain<W>(P, C, m0, mlow, mupp);
+        Strain s0 = kernel_strain<W, KS_E>(P, C, m0, mlow, mupp);
         if (!finite6(s0.E)) { status = DUMP_NONFINITE; goto nocol; }
         double Frho_m0, e_v_m0, p_v_m0;
         evals_ep(s0.E, Frho_m0, e_v_m0, p_v_m0);
         double zvir1 = -1.0, zvir1p = -1.0;
-        // Ebar_last, gradpk_last, gradpkf_last: set at the collapse step only
-        // (between steps it would equal the previous step's, kept below), so
-        // it is not live across the walk
-        Strain last = s0;
+        // The walk forms Ebar only.  The gradients the result keeps are those
+        // of the collapse step (gradpk_last) and of the step before it
+        // (gradpk_prev), formed after the walk over those two steps' bands
+        // (mp, mlow, mupp), as the steps would have formed them.
+        double lastE[6];                // Ebar at the collapse step
         double strain_mat[6];           // the previous step's Ebar
-        double gradpk_prev[3], gradpkf_prev[3];
+        int band_last[3] = {m0, mlow, mupp}, band_prev[3] = {m0, mlow, mupp};
         double Frhoh = Frho_m0, Frho = P.Fbar(m0), e_v = e_v_m0, p_v = p_v_m0;
-        for (int i = 0; i < 6; i++) strain_mat[i] = s0.E[i];
-        for (int L = 0; L < 3; L++) { gradpk_prev[L] = s0.g[L]; gradpkf_prev[L] = s0.gf[L]; }
+        for (int i = 0; i < 6; i++) { strain_mat[i] = s0.E[i]; lastE[i] = s0.E[i]; }
         bool collapsed = false;
         // zvir1p at m0 is -1 (RadialShell.jl:581-583): collapsed at m0 only if ZZon <= -1
         if (-1.0 >= ZZon) {
@@ -546,7 +556,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     int a = first_rad_gt(C, 1, mlow - 1, rlow_mp);
                     mlow = a <= mlow - 1 ? a : mlow;
                 }
-                Strain smp = kernel_strain<W>(P, C, mp, mlow, mupp);
+                Strain smp = kernel_strain<W, KS_E>(P, C, mp, mlow, mupp);
                 if (!finite6(smp.E)) { status = DUMP_NONFINITE; goto nocol; }
                 double Frho_mp, e_v_mp, p_v_mp;
                 evals_ep(smp.E, Frho_mp, e_v_mp, p_v_mp);
@@ -560,13 +570,14 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 if (zvir1p_mp >= ZZon) {
                     m0 = mp + 1;
                     zvir1p = zvir1p_mp;
-                    last = smp;
+                    for (int i = 0; i < 6; i++) lastE[i] = smp.E[i];
+                    band_last[0] = mp; band_last[1] = mlow; band_last[2] = mupp;
                     Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
                     collapsed = true;
                     break;
                 }
                 zvir1 = zvir1p_mp;
-                for (int L = 0; L < 3; L++) { gradpk_prev[L] = smp.g[L]; gradpkf_prev[L] = smp.gf[L]; }
+                band_prev[0] = mp; band_prev[1] = mlow; band_prev[2] = mupp;
                 for (int i = 0; i < 6; i++) strain_mat[i] = smp.E[i];
                 Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
             }
@@ -591,18 +602,26 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         const double RTHL3 = RTHL * RTHL * RTHL, RTHL5 = RTHL3 * RTHL * RTHL;
         const double drad3 = rad3 - rad3p;
         const double dFbar = P.Fbar(m0) - P.Fbar(m0 - 1);
+        const Strain last_g = kernel_strain<W, KS_G>(P, C, band_last[0], band_last[1], band_last[2]);
+        double gradpk_prev[3], gradpkf_prev[3];
+        if (band_prev[0] == band_last[0] && band_prev[1] == band_last[1] && band_prev[2] == band_last[2]) {
+            for (int L = 0; L < 3; L++) { gradpk_prev[L] = last_g.g[L]; gradpkf_prev[L] = last_g.gf[L]; }
+        } else {
+            const Strain t = kernel_strain<W, KS_G>(P, C, band_prev[0], band_prev[1], band_prev[2]);
+            for (int L = 0; L < 3; L++) { gradpk_prev[L] = t.g[L]; gradpkf_prev[L] = t.gf[L]; }
+        }
         double Fbarx, smat[6], gpk[3], gpkf[3];
         if (zvir1 > 0.0 && drad3 != 0.0) {
             double frac = (RTHL3 - rad3p) / drad3;
             Fbarx = P.Fbar(m0 - 1) + frac * dFbar;
-            for (int i = 0; i < 6; i++) smat[i] = last.E[i] + frac * (strain_mat[i] - last.E[i]);
+            for (int i = 0; i < 6; i++) smat[i] = lastE[i] + frac * (strain_mat[i] - lastE[i]);
             for (int L = 0; L < 3; L++) {
-                gpk[L] = last.g[L] + frac * (gradpk_prev[L] - last.g[L]);
-                gpkf[L] = last.gf[L] + frac * (gradpkf_prev[L] - last.gf[L]);
+                gpk[L] = last_g.g[L] + frac * (gradpk_prev[L] - last_g.g[L]);
+                gpkf[L] = last_g.gf[L] + frac * (gradpkf_prev[L] - last_g.gf[L]);
             }
         } else {
             Fbarx = P.Fbar(m0 - 1);
-            for (int i = 0; i < 6; i++) smat[i] = last.E[i];
+            for (int i = 0; i < 6; i++) smat[i] = lastE[i];
             for (int L = 0; L < 3; L++) { gpk[L] = gradpk_prev[L]; gpkf[L] = gradpkf_prev[L]; }
         }
         {   // normalize_strain (RadialShell.jl:209-212)
@@ -689,6 +708,9 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         // ---- zvir_half (RadialShell.jl:821-885) ----
         double zvir_half = -1.0;
         if (RTHL >= 3.0) {
+            // neighbouring radii often land on the same (m0_rc, mlow_rc,
+            // mupp_rc) band; its zvir_rc is then the one already in the max
+            int band_prev[3] = {-1, -1, -1};
             for (int jj = 1; jj <= 10; jj++) {
                 double rcur = RTHL * pow((double)(jj - 1) / 9.0 * 0.5 + 0.5, 1.0 / 3.0);
                 double rupp_rc = rcur + 2.0;
@@ -705,7 +727,9 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                     int f = first_rad_gt(C, 1, m - 1, rcur);
                     m0_rc = f <= m - 1 ? f : m - 1;
                 }
-                Strain src = kernel_strain<W>(P, C, m0_rc, mlow_rc, mupp_rc);
+                if (m0_rc == band_prev[0] && mlow_rc == band_prev[1] && mupp_rc == band_prev[2]) continue;
+                band_prev[0] = m0_rc; band_prev[1] = mlow_rc; band_prev[2] = mupp_rc;
+                Strain src = kernel_strain<W, KS_E>(P, C, m0_rc, mlow_rc, mupp_rc);
                 if (!finite6(src.E)) { status = DUMP_NONFINITE; goto nocol; }
                 double Frho_rc, e_rc, p_rc;
                 evals_ep(src.E, Frho_rc, e_rc, p_rc);
@@ -726,7 +750,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
                 R.gradpkf[o * 3 + L] = gpkf[L];
             }
             // strain_mat = Ebar_last * (Fbarx / Frhoh), column-major 3x3
-            const double *E = last.E;
+            const double *E = lastE;
             double *S = R.strain + o * 9;
             S[0] = E[0] * sfac; S[4] = E[1] * sfac; S[8] = E[2] * sfac;
             S[1] = S[3] = E[3] * sfac;

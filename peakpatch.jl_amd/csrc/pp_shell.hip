@@ -124,13 +124,16 @@ struct GlobalProfile {
     __device__ double S2(int s, int L) const { return (double)f[(PF_S2 + L) * m + s - 1]; }
     __device__ double lap(int s) const { return (double)f[PF_L * m + s - 1]; }
     __device__ bool has_lapd() const { return lapd; }
-    __device__ void strain_raw(int s, float G[3], float Gf[3], float Es[6]) const
+    // centre shell sums are zero (RadialShell.jl:357-364)
+    __device__ void strain_G(int s, float G[3], float Gf[3]) const
     {
-        // centre shell sums are zero (RadialShell.jl:357-364)
         for (int i = 0; i < 3; i++) {
             G[i] = f[(PF_G + i) * m + s - 1];
             Gf[i] = f[(PF_GF + i) * m + s - 1];
         }
+    }
+    __device__ void strain_E(int s, float Es[6]) const
+    {
         float SR[9];
         for (int i = 0; i < 9; i++) SR[i] = f[(PF_SR + i) * m + s - 1];
         Es[0] = SR[0] + SR[0]; Es[1] = SR[4] + SR[4]; Es[2] = SR[8] + SR[8];
