@@ -618,6 +618,21 @@ struct AosProfile {
         G[0] = a.x; G[1] = a.y; G[2] = b.x;
         Gf[0] = b.y; Gf[1] = c.x; Gf[2] = c.y;
     }
+    // shells s and s + 1 (adjacent records, 48 bytes; 4-byte aligned wide loads)
+    __device__ void strain_E2(int s, float Es[12]) const
+    {
+        __builtin_memcpy(Es, se + (int64_t)(s - 1) * PSE, 12 * sizeof(float));
+    }
+    __device__ void strain_G2(int s, float G[6], float Gf[6]) const
+    {
+        float v[12];
+        __builtin_memcpy(v, sg + (int64_t)(s - 1) * PSE, sizeof v);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            G[3 * h] = v[6 * h]; G[3 * h + 1] = v[6 * h + 1]; G[3 * h + 2] = v[6 * h + 2];
+            Gf[3 * h] = v[6 * h + 3]; Gf[3 * h + 1] = v[6 * h + 4]; Gf[3 * h + 2] = v[6 * h + 5];
+        }
+    }
 };
 
 template <bool FULL>

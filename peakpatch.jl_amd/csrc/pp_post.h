@@ -261,12 +261,10 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
     double E[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
     const int4 row = C.wrow[mp - 1];
     const int a = max(mlow, row.y), b = min(mupp, row.z);
-#pragma unroll PP_STRAIN_UNROLL
-    for (int m1 = a + X::lane(); m1 <= b; m1 += W) {
-        const double w = C.wtab[row.x + (m1 - row.y)];
+    // one shell's terms; Es = {SR00+SR00, SR11+SR11, SR22+SR22, SR01+SR10,
+    // SR02+SR20, SR12+SR21}
+    auto add_shell = [&](double w, const float *G, const float *Gf, const float *Es) {
         if constexpr ((KS & KS_G) != 0) {
-            float G[3], Gf[3];
-            P.strain_G(m1, G, Gf);
 #pragma unroll
             for (int L = 0; L < 3; L++) {
                 g[L] += w * (double)G[L];
@@ -274,13 +272,35 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
             }
         }
         if constexpr ((KS & KS_E) != 0) {
-            // Es = {SR00+SR00, SR11+SR11, SR22+SR22, SR01+SR10, SR02+SR20, SR12+SR21}
-            float Es[6];
-            P.strain_E(m1, Es);
             const double hw = 0.5 * w;
 #pragma unroll
             for (int q = 0; q < 6; q++) E[q] -= hw * (double)Es[q];
         }
+    };
+    int m1 = a + X::lane();
+    if constexpr (W == 1) {
+        // One lane per peak: the lanes of a wave read 64 different peaks'
+        // records, so each load instruction touches up to 64 cache lines.
+        // Two shells per iteration halve the instructions: their weights
+        // and records are adjacent (16 bytes and 2 x 24 bytes).  The shells
+        // are still added in order.
+        for (; m1 < b; m1 += 2) {
+            double w2[2];
+            __builtin_memcpy(w2, C.wtab + row.x + (m1 - row.y), sizeof w2);
+            float G[6], Gf[6], Es[12];
+            if constexpr ((KS & KS_G) != 0) P.strain_G2(m1, G, Gf);
+            if constexpr ((KS & KS_E) != 0) P.strain_E2(m1, Es);
+            add_shell(w2[0], G, Gf, Es);
+            add_shell(w2[1], G + 3, Gf + 3, Es + 6);
+        }
+    }
+#pragma unroll PP_STRAIN_UNROLL
+    for (; m1 <= b; m1 += W) {
+        const double w = C.wtab[row.x + (m1 - row.y)];
+        float G[3], Gf[3], Es[6];
+        if constexpr ((KS & KS_G) != 0) P.strain_G(m1, G, Gf);
+        if constexpr ((KS & KS_E) != 0) P.strain_E(m1, Es);
+        add_shell(w, G, Gf, Es);
     }
     Strain s;
     const double scale = mp > 1 ? C.wRnor / P.rad(mp) : C.aRnor;

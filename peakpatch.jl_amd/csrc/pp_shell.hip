@@ -139,6 +139,12 @@ struct GlobalProfile {
         Es[0] = SR[0] + SR[0]; Es[1] = SR[4] + SR[4]; Es[2] = SR[8] + SR[8];
         Es[3] = SR[1] + SR[3]; Es[4] = SR[2] + SR[6]; Es[5] = SR[5] + SR[7];
     }
+    __device__ void strain_E2(int s, float Es[12]) const { strain_E(s, Es); strain_E(s + 1, Es + 6); }
+    __device__ void strain_G2(int s, float G[6], float Gf[6]) const
+    {
+        strain_G(s, G, Gf);
+        strain_G(s + 1, G + 3, Gf + 3);
+    }
 };
 
 __device__ __forceinline__ void gather_cell(const GatherFields &F, long c, long cT, float dx, float dy, float dz,
