@@ -605,6 +605,18 @@ struct AosProfile {
     __device__ double S2(int s, int q) const { return FULL ? (double)x(s)[P_S2 - PSR + q] : 0.0; }
     __device__ double lap(int s) const { return FULL ? (double)x(s)[P_L - PSR] : 0.0; }
     __device__ bool has_lapd() const { return FULL && lapd; }
+    // count, S(3), S2(3) of shell s from one load of its 48-byte record
+    __device__ void sums(int s, double &cnt, double Sv[3], double S2v[3]) const
+    {
+        float v[PSR];
+        __builtin_memcpy(v, x(s), sizeof v);
+        cnt = (double)v[P_N - PSR];
+#pragma unroll
+        for (int L = 0; L < 3; L++) {
+            Sv[L] = FULL ? (double)v[P_S - PSR + L] : 0.0;
+            S2v[L] = FULL ? (double)v[P_S2 - PSR + L] : 0.0;
+        }
+    }
     __device__ void strain_E(int s, float Es[6]) const
     {
         const float2 *q = (const float2 *)(se + (int64_t)(s - 1) * PSE);

@@ -655,12 +655,24 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
 
         // ---- Phase 7 (RadialShell.jl:756-819) ----
         double Srb = 0.0;
-        for (int m1 = 2 + lane; m1 <= m0 - 1; m1 += W) {
-            double ra = P.rad(m1 - 1), rb = P.rad(m1);
-            double r5a = m1 == 2 ? 0.0 : ra * ra * ra * ra * ra, r5b = rb * rb * rb * rb * rb;
-            Srb += 0.5 * (P.Fbar(m1 - 1) + P.Fbar(m1)) * (r5b - r5a);
+        if constexpr (W == 1) {
+            // serially, the inner edge's Fbar and rad^5 are the previous shell's outer ones
+            double fa = P.Fbar(1), r5a = 0.0;
+            for (int m1 = 2; m1 <= m0 - 1; m1++) {
+                const double rb = P.rad(m1), fb = P.Fbar(m1);
+                const double r5b = rb * rb * rb * rb * rb;
+                Srb += 0.5 * (fa + fb) * (r5b - r5a);
+                fa = fb;
+                r5a = r5b;
+            }
+        } else {
+            for (int m1 = 2 + lane; m1 <= m0 - 1; m1 += W) {
+                double ra = P.rad(m1 - 1), rb = P.rad(m1);
+                double r5a = m1 == 2 ? 0.0 : ra * ra * ra * ra * ra, r5b = rb * rb * rb * rb * rb;
+                Srb += 0.5 * (P.Fbar(m1 - 1) + P.Fbar(m1)) * (r5b - r5a);
+            }
+            Srb = X::sum(Srb);
         }
-        Srb = X::sum(Srb);
         if (zvir1 > 0.0 && dZvir != 0.0) {
             double rp = m0 - 1 >= 2 ? P.rad(m0 - 1) : 0.0;
             double rad5p = rp * rp * rp * rp * rp;
@@ -673,9 +685,10 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         if constexpr (FINISH) {
             double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
             for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += W) {
-                double cnt = P.count(m1);
+                double cnt, Sv[3], S2v[3];
+                P.sums(m1, cnt, Sv, S2v);
                 if (cnt > 0.0) {
-                    for (int L = 0; L < 3; L++) { Sb[L] += P.S(m1, L); Sb2[L] += P.S2(m1, L); }
+                    for (int L = 0; L < 3; L++) { Sb[L] += Sv[L]; Sb2[L] += S2v[L]; }
                     nS += cnt;
                 }
             }

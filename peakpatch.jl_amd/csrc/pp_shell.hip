@@ -124,6 +124,11 @@ struct GlobalProfile {
     __device__ double S2(int s, int L) const { return (double)f[(PF_S2 + L) * m + s - 1]; }
     __device__ double lap(int s) const { return (double)f[PF_L * m + s - 1]; }
     __device__ bool has_lapd() const { return lapd; }
+    __device__ void sums(int s, double &cnt, double Sv[3], double S2v[3]) const
+    {
+        cnt = count(s);
+        for (int L = 0; L < 3; L++) { Sv[L] = S(s, L); S2v[L] = S2(s, L); }
+    }
     // centre shell sums are zero (RadialShell.jl:357-364)
     __device__ void strain_G(int s, float G[3], float Gf[3]) const
     {
