@@ -81,22 +81,27 @@ constexpr int NXCD = 8;
 // With eta2, (eta z, eta2 x, eta2 y, eta2 z) go to a second float4 array
 // (q4) for the small-peak gather and the finish pass.  The block stages
 // delta(z, x, y) for 32 x 32 (x, z) in LDS (the transposed read).
+#ifndef PP_PACK_TILE
+#define PP_PACK_TILE 32  // x and z extent of k_pack4's transpose tile (the fields' row runs are T floats)
+#endif
+template <int T>
 __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, const float *__restrict__ ex,
                                                const float *__restrict__ ey, const float *__restrict__ ez,
                                                const float *__restrict__ e2x, const float *__restrict__ e2y,
                                                const float *__restrict__ e2z, float4 *__restrict__ out,
                                                float4 *__restrict__ q4, int n)
 {
-    __shared__ float tile[32][33];
-    const int y = blockIdx.z, x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+    constexpr int R = 256 / T;  // rows per pass of the block
+    __shared__ float tile[T][T + 1];
+    const int y = blockIdx.z, x0 = blockIdx.x * T, z0 = blockIdx.y * T;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const long nn = (long)n * n;
-    for (int r = ty; r < 32; r += 8) {
+    for (int r = ty; r < T; r += R) {
         const int x = x0 + r, z = z0 + tx;
         if (x < n && z < n) tile[r][tx] = d[z + (long)x * n + (long)y * nn];
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
+    for (int r = ty; r < T; r += R) {
         const int z = z0 + r, x = x0 + tx;
         if (x < n && z < n) {
             const long a = x + (long)y * n + (long)z * nn;
@@ -1199,6 +1204,31 @@ static int morton_order(pp_ctx *ctx, const PeakParam *pps, int64_t hi, int n, Li
     return PP_OK;
 }
 
+// 1. packed (delta, delta transposed, eta x, eta y) [and (eta z, eta2)] on the
+// side stream, forked from the context stream as soon as the fields are
+// resident, so that it runs beside the peak set-up and ordering; the gathers
+// wait for event 12.  Event 2 marks the fork (the start of the "prepare"
+// stage).
+int start_pack(pp_ctx *ctx, GatherFields &G)
+{
+    const int n = G.n;
+    const bool pack_q4 = PP_PACK_Q4 && G.e2x != nullptr;
+    PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4) * (pack_q4 ? 2 : 1)));
+    PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[2], 0));
+    constexpr int T = PP_PACK_TILE;
+    dim3 grid((n + T - 1) / T, (n + T - 1) / T, n), block(T, 256 / T);
+    float4 *q4 = pack_q4 ? ctx->d_field_b.as<float4>() + (size_t)n * n * n : nullptr;
+    hipLaunchKernelGGL(k_pack4<T>, grid, block, 0, ctx->side, G.d, G.ex, G.ey, G.ez, G.e2x, G.e2y, G.e2z,
+                       ctx->d_field_b.as<float4>(), q4, n);
+    PP_HIP(hipGetLastError());
+    PP_HIP(hipEventRecord(ctx->ev[12], ctx->side));
+    G.p4 = ctx->d_field_b.as<float4>();
+    G.q4 = q4;
+    G.dT = nullptr;
+    return PP_OK;
+}
+
 // The persistent pipeline on an ordered batch.  export_all keeps the profile
 // for pp_shell_profiles (every peak then goes through k_gather).
 static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, bool export_all,
@@ -1208,24 +1238,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     const int n = G.n;
     if (np > INT_MAX) { set_error("persistent form: at most 2^31-1 peaks per call"); return PP_ERR_CAPACITY; }
     for (double &t : ctx->timing.stage_ms) t = 0.0;
-    PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    // 1. packed (delta, delta transposed, eta x, eta y) [and (eta z, eta2)], on
-    // the side stream while the main stream orders the peaks; joined before
-    // the gathers
-    const bool pack_q4 = PP_PACK_Q4 && G.e2x != nullptr;
-    PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4) * (pack_q4 ? 2 : 1)));
-    {
-        PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[2], 0));
-        dim3 grid((n + 31) / 32, (n + 31) / 32, n), block(32, 8);
-        float4 *q4 = pack_q4 ? ctx->d_field_b.as<float4>() + (size_t)n * n * n : nullptr;
-        hipLaunchKernelGGL(k_pack4, grid, block, 0, ctx->side, G.d, G.ex, G.ey, G.ez, G.e2x, G.e2y, G.e2z,
-                           ctx->d_field_b.as<float4>(), q4, n);
-        PP_HIP(hipGetLastError());
-        PP_HIP(hipEventRecord(ctx->ev[12], ctx->side));
-        G.p4 = ctx->d_field_b.as<float4>();
-        G.q4 = q4;
-        G.dT = nullptr;
-    }
+    // 1. packed fields (start_pack), unless the caller forked the packing
+    // before its own set-up
+    if (!G.p4) PP_CHECK(start_pack(ctx, G));
     // 2. order by m (descending)
     PeakParam *pps = nullptr;
     int32_t *m_sorted = nullptr;

@@ -41,8 +41,9 @@ int upload_fields(pp_ctx *ctx, const pp_fields *f, GatherFields &G);
 int prepare_post_const(pp_ctx *ctx, int n, int nbuff, double alatt, const float *lapd, int8_t *mask,
                        PostConst &C);
 int prepare_batch(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, double rmax2rs,
-                  const int8_t *d_mask, ShellRun &S);
+                  const int8_t *d_mask, ShellRun &S, const pp_results *out = nullptr);
 int run_plain(pp_ctx *ctx, const GatherFields &G, ShellRun &S, PostConst &C, std::vector<int64_t> *offs_out);
+int start_pack(pp_ctx *ctx, GatherFields &G);
 int run_persistent(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C);
 int persistent_profiles(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, std::vector<int32_t> &cnt,
                         std::vector<double> &fbar, std::vector<int64_t> &off);

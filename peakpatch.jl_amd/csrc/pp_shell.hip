@@ -421,8 +421,10 @@ int check_fields(pp_ctx *ctx, const pp_fields *f)
 
 // Device copy of the batch + per-peak parameters.  Fills `S` (pointers into
 // ctx scratch) and returns PP_ERR_ARG if any peak lies outside the grid.
+// Results go to ctx scratch, or straight into the caller's arrays where `out`
+// holds device arrays.
 int prepare_batch(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, double rmax2rs,
-                  const int8_t *d_mask, ShellRun &S)
+                  const int8_t *d_mask, ShellRun &S, const pp_results *out)
 {
     const int64_t np = pb->npeaks;
     const ShellTable &T = ctx->shells;
@@ -465,6 +467,15 @@ int prepare_batch(pp_ctx *ctx, const pp_peak_batch *pb, int n, double alatt, dou
     R.nshells = (int32_t *)p; p += al(np * 4);
     R.m0 = (int32_t *)p; p += al(np * 4);
     R.q = (int32_t *)p; p += al(np * 4);
+    if (out && out->on_device) {
+        double *const dst[13] = {out->RTHL, out->Srb, out->Sbar, out->Sbar2, out->strain, out->gradpk,
+                                 out->gradpkrf, out->Fbarx, out->e_v, out->p_v, out->zvir_half, out->gradpkf,
+                                 out->d2F};
+        for (int t = 0; t < 13; t++)
+            if (dst[t]) *dptr[t] = dst[t];
+        if (out->status) R.status = out->status;
+        if (out->nshells) R.nshells = out->nshells;
+    }
     PP_CHECK(ctx->d_misc.grow(4096));
     int *d_bad = ctx->d_misc.as<int>();
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, d_bad, 1, 0);
@@ -534,7 +545,7 @@ static int copy_results(pp_ctx *ctx, const ShellRun &S, pp_results *out)
         {out->gradpkf, R.gradpkf, 24}, {out->d2F, R.d2F, 8}, {out->status, R.status, 4},
         {out->nshells, R.nshells, 4}};
     for (const Cp &c : cp)
-        if (c.dst) PP_HIP(hipMemcpyAsync(c.dst, c.src, c.b * np, kind, ctx->stream));
+        if (c.dst && c.dst != c.src) PP_HIP(hipMemcpyAsync(c.dst, c.src, c.b * np, kind, ctx->stream));
     unsigned long long *d_h = (unsigned long long *)(ctx->d_misc.as<char>() + 512);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, d_h, 8, 0ull);
     hipLaunchKernelGGL(k_status_hist, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((np + 255) / 256, 1024))),
@@ -574,6 +585,17 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     const size_t N = (size_t)n * n * n;
     GatherFields G;
     PP_CHECK(upload_fields(ctx, fields, G));
+    // the persistent form's field packing starts now, beside the set-up below;
+    // whatever happens next, the side stream has finished with the caller's
+    // fields before this call returns
+    struct SideJoin {
+        hipStream_t s = nullptr;
+        ~SideJoin() { if (s) (void)hipStreamSynchronize(s); }
+    } side_join;
+    if (kernel_form == PP_SHELL_PERSISTENT) {
+        side_join.s = ctx->side;
+        PP_CHECK(start_pack(ctx, G));
+    }
     int8_t *d_mask = nullptr;
     if (mask) {
         if (mask_on_device) d_mask = mask;
@@ -587,7 +609,7 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     PP_CHECK(prepare_post_const(ctx, n, nbuff, alatt, G.lap, d_mask, C));
     PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     ShellRun S;
-    PP_CHECK(prepare_batch(ctx, peaks, n, alatt, rmax2rs, d_mask, S));
+    PP_CHECK(prepare_batch(ctx, peaks, n, alatt, rmax2rs, d_mask, S, out));
     if (kernel_form == PP_SHELL_PERSISTENT) {
         PP_CHECK(run_persistent(ctx, G, S, C));
     } else {
