@@ -687,16 +687,18 @@ __device__ __forceinline__ AosProfile<FULL> profile_at(ProfBuf p, int64_t rb, co
 // ties in position order (a stable radix sort of these keys), so the 64
 // lanes of a wave mostly start their walks at the same shell and step
 // through the same shell bands together.
+// keys[t - lo] = (cmax - m) << cbits | (cmax - crossing), m <= cmax < 2^cbits:
+// ascending keys give m and the crossing descending in 2 * cbits bits.
 __global__ void k_crossing_keys(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
                                 const int64_t *__restrict__ prof_off, const double *__restrict__ rad,
-                                uint32_t *__restrict__ keys)
+                                int cmax, int cbits, uint32_t *__restrict__ keys)
 {
     for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += (int64_t)gridDim.x * blockDim.x) {
         const PeakParam pk = pps[t];
         const int64_t rb = prof_off[t];
         const AosProfile<false> P = profile_at<false>(prof, rb, rad, false);
         const int mc = (pk.flags & 2) ? 0 : crossing_m0(P, pk);
-        keys[t - lo] = (uint32_t)(0xffff - min(pk.m, 0xffff)) << 16 | (uint32_t)(0xffff - min(max(mc, 0), 0xffff));
+        keys[t - lo] = (uint32_t)(cmax - pk.m) << cbits | (uint32_t)(cmax - max(mc, 0));
     }
 }
 
@@ -1405,19 +1407,24 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // its small grids wait for free CUs)
     const int32_t *cross_list = nullptr;
     const int64_t cross_lo = coop_end, cross_hi = small_lo;
+    int cmax = 1, cbits = 1;  // the class's largest m; the keys take 2 * cbits bits
+    for (int x = std::min(std::max(ctx->coop_m, 1), nb - 1); x > small_m; x--)
+        if (hist[x]) { cmax = x; break; }
+    while ((1 << cbits) <= cmax) cbits++;
     auto cross_order = [&]() -> int {
         const int64_t nl = cross_hi - cross_lo;
         if (!PP_POST_SORT || ctx->post_width > 1 || nl <= 64) return PP_OK;
         uint32_t *ka = (uint32_t *)W.ka, *kb = (uint32_t *)W.kb;
         hipLaunchKernelGGL(k_crossing_keys, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, (const PeakParam *)pps,
-                           cross_lo, cross_hi, d_rec, (const int64_t *)d_off, C.rad, ka);
+                           cross_lo, cross_hi, d_rec, (const int64_t *)d_off, C.rad, cmax, cbits, ka);
         hipLaunchKernelGGL(k_iota_from, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, W.iota, nl, (int32_t)cross_lo);
         PP_HIP(hipGetLastError());
         // stable: equal keys keep the caller's order (neighbouring profiles and result rows)
         size_t tmp = 0;
-        PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 32, ctx->stream));
+        PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 2 * cbits,
+                                                  ctx->stream));
         PP_CHECK(ctx->d_cub.grow(tmp));
-        PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 32,
+        PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 2 * cbits,
                                                   ctx->stream));
         cross_list = W.mo;
         return PP_OK;
