@@ -166,7 +166,9 @@ typedef struct {
     int32_t *status;          /* PP_DUMP_* per peak (6: non-finite strain,
                                  where the reference's LAPACK call raises) */
     int32_t *nshells;         /* finalised shells per peak (diagnostic)    */
-    int on_device;            /* 1: the arrays above are device memory     */
+    int on_device;            /* 1: the arrays above are device memory; the
+                                 kernels then write them directly, so they
+                                 must not overlap each other or the inputs */
     int64_t dump_counts[7];   /* out: per status totals (get_dump_counts)  */
 } pp_results;
 
