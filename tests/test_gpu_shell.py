@@ -136,10 +136,12 @@ def mixed_case(ctx):
     ref, st = O.analyse_peaks(O.Grid(d, *eta, eta2=eta2, mask=mask_c, lapd=lapd), ipp1(i, j, k, n), ir2, ZZ, Rf, fc,
                               1.0, tab, O.Shells(rmax), 3, 2.0)
 
-    def run(form):
+    def run(form, device_out=False):
         mask = np.zeros(d.shape, np.int8)
         r = ctx.analyse_peaks(d, eta, i, j, k, Rf, ir2, ZZ, fc, 1.0, eta2=eta2, lapd=lapd, rmax2rs=2.0, nbuff=3,
-                              mask=mask, kernel=form)
+                              mask=mask, kernel=form, device_out=device_out)
+        if device_out:
+            r = {f: (v.cpu().numpy() if hasattr(v, "cpu") else v) for f, v in r.items()}
         return r, mask
     run.autotune = lambda: ctx.autotune_shell(d, eta, i, j, k, Rf, ir2, ZZ, fc, 1.0, eta2=eta2, lapd=lapd,
                                               rmax2rs=2.0)
@@ -158,6 +160,23 @@ def test_forms_agree_on_unsorted_mixed_batch(ctx):
         compare(out[form], ref, st)
         assert np.array_equal(mask, mask_c)
         assert np.array_equal(out[form]["nshells"], out[FORMS[0]]["nshells"])
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_device_results_equal_host_results(ctx, form):
+    """Device result arrays are written by the kernels directly; host arrays
+    are filled by a copy at the end.  Both hold the same bits."""
+    run, ref, st, mask_c = mixed_case(ctx)
+    host, mh = run(form)
+    dev, md = run(form, device_out=True)
+    assert host["dump_counts"] == dev["dump_counts"]
+    for f in host:
+        if f == "dump_counts":
+            continue
+        a, b = np.ascontiguousarray(host[f]), np.ascontiguousarray(dev[f])
+        assert a.shape == b.shape and a.tobytes() == b.tobytes(), f
+    assert np.array_equal(mh, md)
+    compare(dev, ref, st)
 
 
 TUNINGS = [
