@@ -81,6 +81,9 @@ constexpr int NXCD = 8;
 // With eta2, (eta z, eta2 x, eta2 y, eta2 z) go to a second float4 array
 // (q4) for the small-peak gather and the finish pass.  The block stages
 // delta(z, x, y) for 32 x 32 (x, z) in LDS (the transposed read).
+#ifndef PP_PACK_NT
+#define PP_PACK_NT 1  // k_pack4 with non-temporal loads of the eta fields and stores of the packed cells
+#endif
 #ifndef PP_PACK_TILE
 #define PP_PACK_TILE 32  // x and z extent of k_pack4's transpose tile (the fields' row runs are T floats)
 #endif
@@ -105,8 +108,20 @@ __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, cons
         const int z = z0 + r, x = x0 + tx;
         if (x < n && z < n) {
             const long a = x + (long)y * n + (long)z * nn;
+#if PP_PACK_NT
+            // read once and written for later kernels: streamed past the caches
+            typedef float v4f __attribute__((ext_vector_type(4)));
+            const v4f po = {d[a], tile[tx][r], __builtin_nontemporal_load(ex + a), __builtin_nontemporal_load(ey + a)};
+            __builtin_nontemporal_store(po, (v4f *)(out + a));
+            if (q4) {
+                const v4f pq = {__builtin_nontemporal_load(ez + a), __builtin_nontemporal_load(e2x + a),
+                                __builtin_nontemporal_load(e2y + a), __builtin_nontemporal_load(e2z + a)};
+                __builtin_nontemporal_store(pq, (v4f *)(q4 + a));
+            }
+#else
             out[a] = make_float4(d[a], tile[tx][r], ex[a], ey[a]);
             if (q4) q4[a] = make_float4(ez[a], e2x[a], e2y[a], e2z[a]);
+#endif
         }
     }
 }
