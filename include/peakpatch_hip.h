@@ -257,8 +257,9 @@ int pp_pack_halos(pp_ctx *ctx, const pp_results *res, const pp_halo_inputs *in, 
                   float *RTHLmax);
 
 /* Per-stage device time of the last persistent-form call, milliseconds
- * (HIP events), ms[0 .. PP_NSTAGES): ms[0] preparation (transpose, order,
- * offsets), ms[1] wave gather, ms[2] small-peak gather, ms[3] wave post,
+ * (HIP events), ms[0 .. PP_NSTAGES): ms[0] preparation (from the fork of
+ * the field packing, beside the peak set-up, ordering and offsets, to the
+ * start of the gather), ms[1] wave gather, ms[2] small-peak gather, ms[3] wave post,
  * ms[4] lane post, ms[5] finish, ms[6] small-peak post.  The small-peak
  * gather and post run on the context's second stream beside ms[3..5], so
  * the stages overlap.  No counterpart in the reference (its profile=true
