@@ -585,6 +585,7 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     const size_t N = (size_t)n * n * n;
     GatherFields G;
     PP_CHECK(upload_fields(ctx, fields, G));
+    PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));  // shell_ms: from here to the end of the analysis
     // the persistent form's field packing starts now, beside the set-up below;
     // whatever happens next, the side stream has finished with the caller's
     // fields before this call returns
@@ -607,7 +608,6 @@ int pp_analyse_peaks(pp_ctx *ctx, const pp_fields *fields, const pp_peak_batch *
     }
     PostConst C;
     PP_CHECK(prepare_post_const(ctx, n, nbuff, alatt, G.lap, d_mask, C));
-    PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     ShellRun S;
     PP_CHECK(prepare_batch(ctx, peaks, n, alatt, rmax2rs, d_mask, S, out));
     if (kernel_form == PP_SHELL_PERSISTENT) {
