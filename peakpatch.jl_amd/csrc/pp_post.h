@@ -18,6 +18,9 @@
 
 #include <cstdint>
 
+#ifndef PP_PAIR_UNROLL
+#define PP_PAIR_UNROLL 2  // unroll of the two-shell band loop of the one-lane-per-peak posts
+#endif
 #ifndef PP_STRAIN_UNROLL
 #define PP_STRAIN_UNROLL 1  // unroll of the kernel_strain shell loop (overlaps the loads of successive shells)
 #endif
@@ -284,6 +287,7 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
         // Two shells per iteration halve the instructions: their weights
         // and records are adjacent (16 bytes and 2 x 24 bytes).  The shells
         // are still added in order.
+#pragma unroll PP_PAIR_UNROLL
         for (; m1 < b; m1 += 2) {
             double w2[2];
             __builtin_memcpy(w2, C.wtab + row.x + (m1 - row.y), sizeof w2);
