@@ -146,7 +146,7 @@ struct pp_ctx {
     int post_width = 1;  // lanes per peak of the post below coop_m (1, 4, 8, 16)
     int lane_walk = 0;   // lane post: walks longer than this go to the wave post (0: no limit)
     // events: [0,1] whole call, [2..13] persistent-form stage boundaries
-    static constexpr int NEV = 14;
+    static constexpr int NEV = 15;
     hipEvent_t ev[NEV] = {};
     pp::Timing timing;
 };

@@ -81,6 +81,9 @@ constexpr int NXCD = 8;
 // With eta2, (eta z, eta2 x, eta2 y, eta2 z) go to a second float4 array
 // (q4) for the small-peak gather and the finish pass.  The block stages
 // delta(z, x, y) for 32 x 32 (x, z) in LDS (the transposed read).
+#ifndef PP_WAVE_FIRST
+#define PP_WAVE_FIRST 1  // the side stream runs the wave post before the small peaks (needs PP_WAVE_ON_SIDE)
+#endif
 #ifndef PP_PACK_NT
 #define PP_PACK_NT 1  // k_pack4 with non-temporal loads of the eta fields and stores of the packed cells
 #endif
@@ -680,8 +683,9 @@ __device__ __forceinline__ AosProfile<FULL> profile_at(ProfBuf p, int64_t rb, co
 #define PP_POST_SORT 1  // lane post of the wave-gathered peaks in (m, fcrit crossing) order
 #endif
 #ifndef PP_WAVE_ON_SIDE
-#define PP_WAVE_ON_SIDE 1  // the wave post runs on the side stream after the small peaks (beside the lane post)
+#define PP_WAVE_ON_SIDE 1  // the wave post runs on the side stream (beside the lane post)
 #endif
+static_assert(!PP_WAVE_FIRST || PP_WAVE_ON_SIDE, "PP_WAVE_FIRST puts the wave post on the side stream");
 #ifndef PP_SMALL_GRID_CU
 #define PP_SMALL_GRID_CU 32  // workgroups per CU launched for k_small (grid-stride)
 #endif
@@ -1473,9 +1477,25 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // The small peaks (own profile records, own result rows) are gathered and
     // post-processed on the side stream, beside the main stream's work from
     // the fork on; the main stream joins them before the call returns.
+    // the wave post (m > coop_m, one wave per peak) on stream s, between events 13 and 10
+    auto wave_post = [&](hipStream_t s) -> int {
+        PP_HIP(hipEventRecord(ctx->ev[13], s));
+        if (coop_end > 0) {
+            const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * PP_WAVE_GRID_CU));
+            hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, s, (const PeakParam *)pps, (int64_t)0, coop_end,
+                               d_rec, (const int64_t *)d_off, C, S.R);
+            PP_HIP(hipGetLastError());
+        }
+        // on the side stream event 10 marks its end; on the main stream, event 5
+        PP_HIP(hipEventRecord(ctx->ev[s == ctx->side ? 10 : 5], s));
+        return PP_OK;
+    };
     auto side_work = [&]() -> int {
         PP_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
         PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[11], 0));
+        // first, so that the finish pass (which needs its statuses) can start
+        // as soon as the lane post ends
+        if (PP_WAVE_FIRST) PP_CHECK(wave_post(ctx->side));
         if (nblk > 0) {
             SmallArgs A;
             A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
@@ -1492,7 +1512,8 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         }
         PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
         PP_CHECK(post_lane(small_lo, np, true));
-        if (!PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
+        if (PP_WAVE_FIRST) PP_HIP(hipEventRecord(ctx->ev[14], ctx->side));
+        else if (!PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
         return PP_OK;
     };
     PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[12], 0));  // the packed fields
@@ -1521,18 +1542,12 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     PP_CHECK(cross_order());
     PP_CHECK(side_work());  // fork after the wave gather (measured best: beside the posts, not the gather)
-    // 5. posts: the wave post on the side stream after the small peaks' work
-    // (PP_WAVE_ON_SIDE), beside the lane post on the main stream
+    // 5. posts: the wave post on the side stream (PP_WAVE_ON_SIDE; before the
+    // small peaks' work with PP_WAVE_FIRST, else after it), beside the lane
+    // post on the main stream
     hipStream_t wave_stream = PP_WAVE_ON_SIDE ? ctx->side : ctx->stream;
     if (PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-    PP_HIP(hipEventRecord(ctx->ev[13], wave_stream));
-    if (coop_end > 0) {
-        const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * PP_WAVE_GRID_CU));
-        hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, wave_stream, (const PeakParam *)pps, (int64_t)0, coop_end,
-                           d_rec, (const int64_t *)d_off, C, S.R);
-        PP_HIP(hipGetLastError());
-    }
-    PP_HIP(hipEventRecord(ctx->ev[PP_WAVE_ON_SIDE ? 10 : 5], wave_stream));
+    if (!PP_WAVE_FIRST) PP_CHECK(wave_post(wave_stream));
     if (ctx->post_width <= 1) {
         PP_CHECK(post_lane(coop_end, small_lo, false));
     } else if (small_lo > coop_end) {
@@ -1568,7 +1583,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     }
     PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     // join: everything after this call on the context stream sees the small peaks' results
-    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[10], 0));
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[PP_WAVE_FIRST ? 14 : 10], 0));
     if (cnt) {
         // profile export: counts and Fbar per shell in the caller's order
         std::vector<float> hr((size_t)tot * PSR);
@@ -1603,8 +1618,10 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // main stream: 2 start | prepare | 3 | k_gather | 4 | [wave post] | 5 | lane post |
     // 6 | finish | 7 | join | 8;  side: 11 (fork) | k_small | 9 | small post | 13 |
     // [wave post] | 10 (the wave post runs on one of the two)
-    const int from[Timing::NSTAGE] = {2, 3, 11, PP_WAVE_ON_SIDE ? 13 : 4, 5, 6, 9};
-    const int to[Timing::NSTAGE] = {3, 4, 9, PP_WAVE_ON_SIDE ? 10 : 5, 6, 7, PP_WAVE_ON_SIDE ? 13 : 10};
+    // with PP_WAVE_FIRST the side stream runs 11 (fork) | wave post | 10 | k_small | 9 | small post | 14
+    const int from[Timing::NSTAGE] = {2, 3, PP_WAVE_FIRST ? 10 : 11, PP_WAVE_ON_SIDE ? 13 : 4, 5, 6, 9};
+    const int to[Timing::NSTAGE] = {3, 4, 9, PP_WAVE_ON_SIDE ? 10 : 5, 6, 7,
+                                    PP_WAVE_FIRST ? 14 : (PP_WAVE_ON_SIDE ? 13 : 10)};
     for (int q = 0; q < Timing::NSTAGE; q++) {
         float t = 0.0f;
         (void)hipEventElapsedTime(&t, ctx->ev[from[q]], ctx->ev[to[q]]);
