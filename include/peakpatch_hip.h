@@ -38,7 +38,7 @@ extern "C" {
 #define PP_ERR_TIMEOUT (-8)   /* serving wait timed out                */
 #define PP_ERR_INJECTED (-9)  /* fault injected for failure tests      */
 
-/* smooth_field window selector (Filters.jl:143-186, wsmooth) */
+/* smooth_field window selector (Filters.jl:18-36, 79-106, wsmooth) */
 #define PP_WINDOW_GAUSSIAN 0  /* exp(-k^2 R^2 / 8)          */
 #define PP_WINDOW_TOPHAT 1    /* 3 (sin x - x cos x) / x^3  */
 #define PP_WINDOW_K2GAUSS 3   /* k^2 exp(-k^2 R^2 / 8)      */
@@ -72,6 +72,16 @@ int pp_ctx_synchronize(pp_ctx *ctx);
  * device work (input synthesis, copies) on the same queue.  No counterpart in
  * the reference (CUDA.jl's task-local stream). */
 int pp_ctx_stream(pp_ctx *ctx, void **stream);
+/* Stream ordering with a caller's HIP stream (NULL = the legacy default
+ * stream), for callers that produce or consume device operands on their own
+ * queue (the task-local stream of CUDA.jl / AMDGPU.jl, torch's current
+ * stream).  pp_ctx_wait_stream: work the context launches next runs after
+ * the work queued on `stream` so far.  pp_ctx_signal_stream: work queued on
+ * `stream` next runs after everything the context has launched so far.
+ * Both only enqueue waits; neither blocks the host.  The reference has no
+ * counterpart: CUDA.jl orders its own task-local stream implicitly. */
+int pp_ctx_wait_stream(pp_ctx *ctx, void *stream);
+int pp_ctx_signal_stream(pp_ctx *ctx, void *stream);
 /* Free / total device memory as seen by this context (for admission). */
 int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 
@@ -89,14 +99,14 @@ int pp_shell_table_info(pp_ctx *ctx, int64_t *ncells, int32_t *nshells);
 
 /* ---------------------------------------------------------- filter bank */
 /* In-place or out-of-place k-space window multiply of smooth_field
- * (src/HaloFinder/Filters.jl:159-184; ext/CUDAExt.jl:2959-3000): every
+ * (src/HaloFinder/Filters.jl:79-106; ext/CUDAExt.jl:2959-3000): every
  * element of delta_k (complex64, (n/2+1) x n x n, x fastest) is multiplied by
  * Float32(W(|k| Rf)) computed in Float64; k = 0 is left unscaled; Nyquist kept.
  * scale multiplies every element afterwards (1 for the bare window). */
 int pp_window_multiply(pp_ctx *ctx, const void *delta_k, void *out, int on_device, int n,
                        double boxsize, double Rf, int wsmooth, float scale);
 
-/* smooth_field (Filters.jl:159-186): window multiply + normalised C2R.
+/* smooth_field (Filters.jl:79-106): window multiply + normalised C2R.
  * delta_k complex64 (n/2+1)xnxn; out Float32 n^3. */
 int pp_smooth_field(pp_ctx *ctx, const void *delta_k, float *out, int on_device, int n,
                     double boxsize, double Rf, int wsmooth);

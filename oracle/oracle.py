@@ -205,7 +205,7 @@ def window_multiply(delta_k: np.ndarray, n: int, boxsize: float, Rf: float, wsmo
 
 
 def smooth_field(delta_k: np.ndarray, n: int, boxsize: float, Rf: float, wsmooth: int) -> np.ndarray:
-    """smooth_field (Filters.jl:159-186): window multiply (C) + FP64 pocketfft irfft.
+    """smooth_field (Filters.jl:79-106): window multiply (C) + FP64 pocketfft irfft.
     Returns float64 (z, y, x) C-order array == Julia (x, y, z) column-major."""
     wk = window_multiply(delta_k, n, boxsize, Rf, wsmooth)
     return np.fft.irfftn(wk.astype(np.complex128), s=(n, n, n), axes=(0, 1, 2))

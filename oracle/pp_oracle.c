@@ -222,7 +222,7 @@ double orc_interp_c(const float *data, int nx, int ny, int nz, double X1, double
 }
 
 /* ------------------------------------------------------------------------ */
-/* Windows  (src/HaloFinder/Filters.jl:98-116)                               */
+/* Windows  (src/HaloFinder/Filters.jl:18-36)                                */
 /* ------------------------------------------------------------------------ */
 double orc_tophat_window(double kR)
 {
@@ -232,7 +232,7 @@ double orc_tophat_window(double kR)
 double orc_gaussian_window(double kR) { return exp(-(kR * kR) / 2.0); }
 double orc_gaussian_window_fortran(double kR) { return orc_gaussian_window(kR / 2.0); }
 
-/* k-space multiply of smooth_field (Filters.jl:159-186) without the FFT:
+/* k-space multiply of smooth_field (Filters.jl:79-106) without the FFT:
  * out[ix,iy,iz] = in * Float32(W(k*Rf)), k computed in Float64, k=0 left
  * unscaled, Nyquist kept.  in/out are interleaved complex64, shape
  * (n/2+1, n, n), x fastest.  in == out is allowed. */

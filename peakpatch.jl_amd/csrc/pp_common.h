@@ -148,12 +148,12 @@ struct pp_ctx {
     // events: [0,1] whole call, [2..13] persistent-form stage boundaries
     static constexpr int NEV = 15;
     hipEvent_t ev[NEV] = {};
+    hipEvent_t ev_join = nullptr, ev_join2 = nullptr;  // order the context's streams against a caller's
     pp::Timing timing;
 };
 
 namespace pp {
 int fft_prepare(pp_ctx *ctx, int n);
-int shell_groups_build(pp_ctx *ctx);  // pp_persist.hip
 void persist_release(pp_ctx *ctx);    // pp_persist.hip: drops the context's gather tables
 int ensure_device(pp_ctx *ctx);
 }  // namespace pp

@@ -114,10 +114,14 @@ int pp_ctx_create(int device, pp_ctx **out)
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     for (int i = 0; i < pp_ctx::NEV && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming);
     if (e != hipSuccess) {
         set_error("context creation on device %d failed: %s", device, hipGetErrorString(e));
         for (auto &ev : c->ev)
             if (ev) (void)hipEventDestroy(ev);
+        if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+        if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
         if (c->side) (void)hipStreamDestroy(c->side);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -152,6 +156,8 @@ int pp_ctx_destroy(pp_ctx *ctx)
     for (auto &b : ctx->d_in) b.release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_join2) (void)hipEventDestroy(ctx->ev_join2);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -171,6 +177,36 @@ int pp_ctx_stream(pp_ctx *ctx, void **stream)
 {
     PP_ARG(ctx && stream, "NULL argument");
     *stream = (void *)ctx->stream;
+    return PP_OK;
+}
+
+int pp_ctx_wait_stream(pp_ctx *ctx, void *stream)
+{
+    PP_ARG(ctx, "ctx is NULL");
+    PP_CHECK(ensure_device(ctx));
+    hipStream_t s = (hipStream_t)stream;
+    if (s == ctx->stream) return PP_OK;
+    // the event is recorded on the caller's queue and waited on by both of
+    // the context's streams, so whatever either launches next runs after
+    // the work the caller has queued so far
+    PP_HIP(hipEventRecord(ctx->ev_join, s));
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_join, 0));
+    return PP_OK;
+}
+
+int pp_ctx_signal_stream(pp_ctx *ctx, void *stream)
+{
+    PP_ARG(ctx, "ctx is NULL");
+    PP_CHECK(ensure_device(ctx));
+    hipStream_t s = (hipStream_t)stream;
+    if (s == ctx->stream) return PP_OK;
+    // the side stream's work joins the main stream, whose tail the caller's
+    // queue then waits for (two events: each wait keeps its own record)
+    PP_HIP(hipEventRecord(ctx->ev_join2, ctx->side));
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
+    PP_HIP(hipEventRecord(ctx->ev_join, ctx->stream));
+    PP_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
     return PP_OK;
 }
 
@@ -259,7 +295,6 @@ int pp_build_shell_tables(pp_ctx *ctx, int rmax)
     PP_HIP(hipMemcpy(T.d_shell_of_r2.ptr, shell_of_r2.data(), shell_of_r2.size() * sizeof(int32_t),
                      hipMemcpyHostToDevice));
     T.rmax = rmax;
-    PP_CHECK(pp::shell_groups_build(ctx));
     return PP_OK;
 }
 

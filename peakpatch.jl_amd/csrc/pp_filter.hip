@@ -1,7 +1,7 @@
 // pp_filter.hip -- k-space filter-bank multiply, rocFFT transforms and the
 // 27-point local-maximum peak stencil.
 //
-// Reference: smooth_field (src/HaloFinder/Filters.jl:159-186), find_peaks
+// Reference: smooth_field (src/HaloFinder/Filters.jl:79-106), find_peaks
 // (src/HaloFinder/PeakFind.jl:15-78), driver loop (src/Pipeline.jl:160-195,
 // src/MultiResolution.jl:885-907), GPU counterpart peak_find_tile_gpu
 // (ext/CUDAExt.jl:2959-3197).

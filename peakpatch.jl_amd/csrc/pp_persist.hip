@@ -1642,6 +1642,6 @@ int persistent_profiles(pp_ctx *ctx, GatherFields G, ShellRun &S, PostConst &C, 
     return run_persistent_impl(ctx, G, S, C, true, &cnt, &fbar, &off);
 }
 
-int shell_groups_build(pp_ctx *) { return PP_OK; }
+
 
 }  // namespace pp

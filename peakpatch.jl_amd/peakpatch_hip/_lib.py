@@ -70,6 +70,8 @@ SIGNATURES = {
     "pp_ctx_synchronize": (_i, [_vp]),
     "pp_ctx_mem_info": (_i, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "pp_ctx_stream": (_i, [_vp, C.POINTER(_vp)]),
+    "pp_ctx_wait_stream": (_i, [_vp, _vp]),
+    "pp_ctx_signal_stream": (_i, [_vp, _vp]),
     "pp_upload_collapse_table": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d]),
     "pp_build_shell_tables": (_i, [_vp, _i]),
     "pp_shell_table_info": (_i, [_vp, C.POINTER(_i64), C.POINTER(C.c_int32)]),

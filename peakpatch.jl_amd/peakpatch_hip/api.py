@@ -112,13 +112,30 @@ class Context:
         L.check(self.lib.pp_ctx_stream(self.h, C.byref(s)))
         return int(s.value or 0)
 
+    def _call(self, fn, on_device):
+        """Run one library call.  With device operands the context's streams
+        first wait for the work queued so far on torch's current stream
+        (where the caller's tensors were produced), and afterwards that
+        stream waits for the context's (where they were written), so torch
+        work on either side of the call is ordered with it."""
+        if not on_device:
+            return L.check(fn())
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pp_ctx_wait_stream(self.h, s))
+        rc = L.check(fn())
+        L.check(self.lib.pp_ctx_signal_stream(self.h, s))
+        return rc
+
     def fill_noise(self, n: int, seed: int, out=None):
         """Threefry white noise of an n^3 tile as a CUDA float32 tensor
-        (RandomField.jl:269-306), generated on the context's stream."""
+        (RandomField.jl:269-306), generated on the context's stream and
+        ordered before later work on torch's current stream."""
         import torch
         if out is None:
             out = torch.empty((n, n, n), dtype=torch.float32, device=f"cuda:{self.device}")
-        L.check(self.lib.pp_fill_noise_threefry(self.h, out.data_ptr(), n * n * n, int(seed) & 0xFFFFFFFFFFFFFFFF))
+        self._call(lambda: self.lib.pp_fill_noise_threefry(self.h, out.data_ptr(), n * n * n,
+                                                           int(seed) & 0xFFFFFFFFFFFFFFFF), True)
         return out
 
     def synchronize(self):
@@ -172,7 +189,7 @@ class Context:
         return out
 
     def smooth_field(self, delta_k, n: int, boxsize: float, Rf: float, wsmooth: int) -> np.ndarray:
-        """smooth_field (Filters.jl:159-186) on the GPU; host in, host out."""
+        """smooth_field (Filters.jl:79-106) on the GPU; host in, host out."""
         a = np.ascontiguousarray(delta_k, np.complex64)
         out = np.empty((n, n, n), np.float32)
         L.check(self.lib.pp_smooth_field(self.h, a.ctypes.data, out.ctypes.data, 0, n, boxsize, Rf, wsmooth))
@@ -192,7 +209,7 @@ class Context:
         n3, n2, n1 = delta_s.shape
         cap = capacity if capacity is not None else max(1, min(max_peaks, (n1 - 2 * nbuff) * (n2 - 2 * nbuff) * (n3 - 2 * nbuff)))
         return self._run_peaks(lambda pk: self.lib.pp_find_peaks_field(
-            self.h, dp, mp, dev, n1, n2, n3, nbuff, fcrit, max_peaks, C.byref(pk)), cap, False)
+            self.h, dp, mp, dev, n1, n2, n3, nbuff, fcrit, max_peaks, C.byref(pk)), cap, False, dev)
 
     def find_peaks(self, delta, boxsize: float, Rf, fcrit, wsmooth: int = 1, want_lapd: bool = False,
                    nbuff: int = 1, max_peaks_per_filter: int = 2_000_000, mask=None,
@@ -212,15 +229,15 @@ class Context:
         cap = capacity if capacity is not None else max(1, min(inner, len(Rf) * max_peaks_per_filter, 1 << 24))
         return self._run_peaks(lambda pk: self.lib.pp_find_peaks(
             self.h, dp, dev, n, boxsize, Rf.ctypes.data, fc.ctypes.data, len(Rf), wsmooth, int(want_lapd),
-            nbuff, max_peaks_per_filter, mp, C.byref(pk)), cap, want_lapd)
+            nbuff, max_peaks_per_filter, mp, C.byref(pk)), cap, want_lapd, dev)
 
-    def _run_peaks(self, call, cap, want_lapd) -> PeakList:
+    def _run_peaks(self, call, cap, want_lapd, on_device) -> PeakList:
         bufs = {k: np.zeros(cap, np.int32) for k in ("i", "j", "k", "filt")}
         bufs["delta_s"] = np.zeros(cap, np.float32)
         bufs["lapd_s"] = np.zeros(cap, np.float32) if want_lapd else None
         pk = L.Peaks(cap, 0, 0, *(bufs[k].ctypes.data if bufs[k] is not None else None
                                   for k in ("i", "j", "k", "filt", "delta_s", "lapd_s")))
-        L.check(call(pk))
+        self._call(lambda: call(pk), on_device)
         c = pk.count
         return PeakList(bufs["i"][:c].copy(), bufs["j"][:c].copy(), bufs["k"][:c].copy(),
                         bufs["filt"][:c].copy(), bufs["delta_s"][:c].copy(),
@@ -297,8 +314,9 @@ class Context:
                 if not (isinstance(mask, np.ndarray) and mask.dtype == np.int8 and mask.flags.c_contiguous):
                     raise ValueError("mask must be a C-contiguous int8 numpy array (updated in place)")
                 mp = mask.ctypes.data
-        L.check(self.lib.pp_analyse_peaks(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, nbuff, mp, mdev,
-                                          kernel, C.byref(r)))
+        self._call(lambda: self.lib.pp_analyse_peaks(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, nbuff, mp,
+                                                     mdev, kernel, C.byref(r)),
+                   bool(f.on_device or b.on_device or mdev or device_out))
         # strain is column-major per peak: [p, L + 3K] -> (p, L, K)
         if device_out:
             out["strain"] = out["strain"].reshape(n, 3, 3).transpose(1, 2)
@@ -314,7 +332,8 @@ class Context:
         f, keep = self._fields(delta, eta, eta2, lapd)
         b, keepb = self._batch(i, j, k, Rf, ir2min, ZZon, fcrit)
         choice = C.c_int()
-        L.check(self.lib.pp_autotune_shell(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, C.byref(choice)))
+        self._call(lambda: self.lib.pp_autotune_shell(self.h, C.byref(f), C.byref(b), alatt, rmax2rs,
+                                                      C.byref(choice)), bool(f.on_device or b.on_device))
         return self.tunables()
 
     def shell_profiles(self, delta, eta, i, j, k, Rf, alatt: float, eta2=None, lapd=None,
@@ -325,6 +344,7 @@ class Context:
         m = np.zeros(n, np.int32)
         ns = np.zeros((n, max_shells), np.int32)
         fb = np.zeros((n, max_shells), np.float64)
-        L.check(self.lib.pp_shell_profiles(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, kernel, max_shells,
-                                           m.ctypes.data, ns.ctypes.data, fb.ctypes.data))
+        self._call(lambda: self.lib.pp_shell_profiles(self.h, C.byref(f), C.byref(b), alatt, rmax2rs, kernel,
+                                                      max_shells, m.ctypes.data, ns.ctypes.data, fb.ctypes.data),
+                   bool(f.on_device))
         return m, ns, fb

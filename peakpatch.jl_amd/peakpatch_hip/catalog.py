@@ -40,8 +40,8 @@ def pack_halos(ctx, res: dict, bdev: dict, alatt: float, extended: bool, z_max: 
     # size the output by the collapsed count (the pack call reports it)
     cap = int((res["RTHL"] > 0).sum().item())
     out = torch.empty((max(cap, 1), nf), dtype=torch.float32, device=bdev["x"].device)
-    L.check(ctx.lib.pp_pack_halos(ctx.h, C.byref(r), C.byref(h), alatt, z_max, 1 if extended else 0,
-                                  out.data_ptr(), cap, 1, C.byref(nh), C.byref(rmax)))
+    ctx._call(lambda: ctx.lib.pp_pack_halos(ctx.h, C.byref(r), C.byref(h), alatt, z_max, 1 if extended else 0,
+                                            out.data_ptr(), cap, 1, C.byref(nh), C.byref(rmax)), True)
     return out[: nh.value], float(rmax.value)
 
 

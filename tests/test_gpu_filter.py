@@ -171,3 +171,37 @@ def test_threefry_noise_kernel_matches_host(ctx):
     # differ in the last bits, which the Float32 cast almost always hides
     assert np.abs(got - ref).max() <= 2e-6
     assert (got == ref).mean() > 0.99
+
+
+def test_fill_noise_is_ordered_before_torch_work(ctx):
+    """The noise kernel runs on the context's stream; torch work queued right
+    after the call (no explicit synchronisation) must see the finished draw."""
+    import torch
+    n = 256
+    ref = ctx.fill_noise(n, 99)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ref = ref.cpu()
+    for _ in range(3):
+        got = ctx.fill_noise(n, 99).clone()
+        assert torch.equal(got.cpu(), ref)
+
+
+def test_tile_synthesis_with_context_is_deterministic(ctx):
+    """make_tile_torch(ctx=...) with no synchronisation between the noise
+    kernel and torch's FFTs gives the same fields as a synchronised draw."""
+    import torch
+    from peakpatch_hip import synth
+    n, box = 128, 100.0
+    ref_noise = ctx.fill_noise(n, 7)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t_ref = synth.make_tile_torch(n, box, 7, ilpt=1, want_lapd=False, device="cuda")
+    t_ctx = synth.make_tile_torch(n, box, 7, ilpt=1, want_lapd=False, device="cuda", ctx=ctx)
+    t_ctx2 = synth.make_tile_torch(n, box, 7, ilpt=1, want_lapd=False, device="cuda", ctx=ctx)
+    torch.cuda.synchronize()
+    assert torch.equal(t_ctx.delta, t_ctx2.delta)
+    # the torch Threefry restatement and the kernel draw the same noise up
+    # to the last bits of Box-Muller, so the fields agree to float32 noise
+    assert torch.allclose(t_ctx.delta, t_ref.delta, rtol=0, atol=1e-4 * float(t_ref.delta.abs().max()))
+    assert ref_noise.shape == (n, n, n)
