@@ -72,12 +72,28 @@ def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
     return res, walls, kern, stages
 
 
-def cpu_baseline(cfg, tile, b, ct, seconds: float):
-    """Oracle (FP64 C restatement, OpenMP dynamic over peaks) on a stratified
-    sample of the tile's peaks, sized to about `seconds` of CPU time."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, tile, b, ct, seconds: float, runs: int = 5):
+    """Oracle (FP64 C restatement, OpenMP dynamic over peaks) on the tile's
+    peaks: the median of `runs` timings on every CPU this process may use
+    (a stratified sample sized to about `seconds` per run), and one run on a
+    single core over a smaller stratified sample."""
     import numpy as np
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    avail = len(os.sched_getaffinity(0))
+    # OMP_NUM_THREADS states the CPU share a job is given where one is set
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, share) if share > 0 else avail
     host = lambda t: t.detach().cpu().numpy()
     delta = host(tile.delta)
     eta = tuple(host(a) for a in tile.eta)
@@ -90,22 +106,31 @@ def cpu_baseline(cfg, tile, b, ct, seconds: float):
     n = cfg.n
     ipp = (b.i.astype(np.int64) + 1) + b.j.astype(np.int64) * n + b.k.astype(np.int64) * n * n
 
-    def run(sel):
+    def run(sel, nth):
         t0 = time.perf_counter()
         O.analyse_peaks(grid, ipp[sel], b.ir2min[sel], b.ZZon[sel], b.Rf[sel], b.fcrit[sel], cfg.alatt, tab, sh,
-                        cfg.nbuff_eff, cfg.rmax2rs, threads)
+                        cfg.nbuff_eff, cfg.rmax2rs, nth)
         return time.perf_counter() - t0
 
     N = len(ipp)
     stride = max(1, N // 2000)
-    t = run(np.arange(0, N, stride))
+    t = run(np.arange(0, N, stride), threads)
     est_all = t * stride
     stride2 = max(1, int(math.ceil(est_all / seconds)))
     sel = np.arange(0, N, stride2)
-    t = run(sel)
+    times = [run(sel, threads) for _ in range(runs)]
+    t = float(np.median(times))
+    # one core: a sample sized from the multi-core rate (about `seconds`)
+    stride1 = max(1, int(math.ceil(N * t / len(sel) * threads / seconds)))
+    sel1 = np.arange(0, N, stride1)
+    t1 = run(sel1, 1)
     return {"value": len(sel) / t, "unit": "peaks/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride2}th peak of the same tile ({len(sel)} of {N}, all filters), "
-                      f"{t:.1f} s, FP64 oracle restating RadialShell.analyse_peak"}
+            "sample": f"every {stride2}th peak of the same tile ({len(sel)} of {N}, all filters), median of "
+                      f"{runs} runs ({min(times):.2f}-{max(times):.2f} s), FP64 oracle restating "
+                      f"RadialShell.analyse_peak, OpenMP dynamic over peaks",
+            "runs_s": [round(x, 3) for x in times], "host_cpus": avail, "cpu_model": cpu_model(),
+            "one_core": {"value": len(sel1) / t1, "unit": "peaks/s",
+                         "sample": f"every {stride1}th peak ({len(sel1)}), {t1:.1f} s"}}
 
 
 def read_traffic(path, head, kernel):

@@ -1,5 +1,6 @@
 """Parity at the full light-cone tile: nmesh = 512, box 600 Mpc/h, 2LPT, the
-20-filter bank (about 2 M peaks), set up as the benchmark sets it up.
+25-filter bank 1-30 Mpc/h (about 2 M peaks), set up as the benchmark sets
+it up.
 
 Properties that hold at any size:
 - the persistent form is deterministic: two runs give bit-identical
@@ -101,6 +102,14 @@ def test_c3_full_tile():
         assert c["rthl_within_one_bin"] >= 0.999
         assert c["mass_within_0.5pct"] >= 0.999
         assert c["mask_mismatch_frac"] < 1e-3
+        cq = compare_catalogs(q, ref, q["status"], st, rad_of(cfg.nhunt), cfg.alatt, cfg.cosmology.Om,
+                              cfg.cosmology.h)
+        cq["mask_mismatch_frac"] = float(((mq != 0) != (mask_c != 0)).sum() / max(1, masked.sum()))
+        print("[c3] plain vs oracle", json.dumps({k: v for k, v in cq.items() if k != "fields"}), flush=True)
+        assert cq["status_agreement"] >= 0.9999
+        assert cq["rthl_within_one_bin"] >= 0.999
+        assert cq["mass_within_0.5pct"] >= 0.999
+        assert cq["mask_mismatch_frac"] < 1e-3
     finally:
         ctx.close()
 

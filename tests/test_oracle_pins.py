@@ -324,3 +324,94 @@ def test_chi_and_chi_to_z():
         assert c2z(O.chi(z, Om, OL, h)) == pytest.approx(z, abs=5e-3)
     assert c2z(-1.0) == 0.0 and c2z(1e9) == 5.0
     assert c2z.peak_redshift((0, 0, 0), (0, 0, O.chi(0.5, Om, OL, h))) == pytest.approx(0.5, abs=5e-3)
+
+
+# ---- analyse_peak outputs no reference vector fixes: analytic cases ----
+# (tests/pin_cases.py states each expectation and why it holds)
+import pin_cases as PC  # noqa: E402
+
+
+def _run_case(case, table, bounds, fcrit, ZZon, mask=None):
+    g = O.Grid(case.delta, *case.eta, eta2=case.eta2, mask=mask, lapd=case.lapd)
+    r, st, prof = O.analyse_peak(g, case.ipp, 1.0, case.ir2min, ZZon, 4.0, O.Table(table, *bounds),
+                                 O.Shells(case.rmax), nbuff=case.nbuff, rmax2rs=0.0, fcrit=fcrit,
+                                 profile_cap=2000)
+    return r, st, prof
+
+
+def _check_common(case, r, R, Fbarx, inside_r2):
+    ex = PC.expect_common(case, R, Fbarx, inside_r2)
+    st = np.asarray(r["strain"]).reshape(3, 3, order="F")
+    assert np.allclose(st, ex["strain"], rtol=1e-10, atol=1e-12)
+    assert r["e_v"] == pytest.approx(ex["e_v"], rel=1e-10)
+    assert r["p_v"] == pytest.approx(ex["p_v"], rel=1e-9)
+    # kernel-weighted gradient of a linear field: its slope, up to the
+    # kernel's discretisation error on the lattice
+    assert np.allclose(r["gradpk"], ex["gradpk"], rtol=1e-4)
+    # quirk Q6: the permuted read rotates the slopes
+    assert np.allclose(r["gradpkf"], ex["gradpkf"], rtol=1e-4)
+    assert np.allclose(r["gradpkf"], np.asarray(r["gradpk"])[[1, 2, 0]], rtol=1e-9)
+    assert np.allclose(r["Sbar2"], ex["Sbar2"], rtol=1e-12)
+    assert r["d2F"] == pytest.approx(ex["d2F"], rel=1e-12)
+
+
+@pytest.mark.parametrize("fcrit", [1.0, 1.686, 3.0])
+def test_pin_constant_table_collapse(fcrit):
+    """Every query collapses: R_TH = radius of the last shell with
+    Fbar >= fcrit, Fbarx = its Fbar, zvir_half = table value - 1, and the mask
+    holds exactly the cells within R_TH inside the nbuff frame."""
+    case = PC.bump_case()
+    t, b = PC.constant_table(1.8)
+    mask = np.zeros(case.delta.shape, np.int8)
+    r, st, prof = _run_case(case, t, b, fcrit, 1.5, mask)
+    assert st == 0
+    rad, fb = PC.fbar_profile(case)
+    mc = PC.crossing(fb, fcrit)
+    assert r["RTHL"] == rad[mc - 1] and r["RTHL"] >= 3.0
+    assert r["Fbarx"] == pytest.approx(fb[mc - 1], rel=1e-12)
+    assert r["zvir_half"] == pytest.approx(float(np.float32(1.8)) - 1.0, rel=1e-12)
+    _check_common(case, r, r["RTHL"], r["Fbarx"], rad[mc - 1] ** 2)
+    sel, _ = PC.inside_cells(case, r["RTHL"], frame=case.nbuff)
+    assert np.array_equal(mask.astype(bool), sel)
+
+
+@pytest.mark.parametrize("a,b,fcrit,ZZon", [(1.0, 1.0, 1.0, 1.5), (1.2, 0.8, 1.686, 1.7), (0.9, 1.5, 1.0, 2.0)])
+def test_pin_linear_table_interpolates_rthl(a, b, fcrit, ZZon):
+    """zvir = a + b log10 F: the walk stops at the outermost shell inside the
+    fcrit crossing with zvir >= 1 + z, R_TH interpolates zvir linearly in r^3
+    to 1 + z, Fbarx interpolates Fbar the same way, and zvir_half is the
+    largest zvir - 1 over ten radii from R_TH / 2^(1/3) to R_TH."""
+    case = PC.bump_case()
+    t, bounds = PC.linear_table(a, b)
+    mask = np.zeros(case.delta.shape, np.int8)
+    r, st, prof = _run_case(case, t, bounds, fcrit, ZZon, mask)
+    assert st == 0
+    ex = PC.expect_linear(case, a, b, fcrit, ZZon)
+    rad, fb = PC.fbar_profile(case)
+    assert prof["m0"] == ex["inside"] + 1
+    assert r["RTHL"] == pytest.approx(ex["RTHL"], rel=1e-12) and r["RTHL"] != rad[ex["inside"]]
+    assert r["Fbarx"] == pytest.approx(ex["Fbarx"], rel=1e-12)
+    assert r["zvir_half"] == pytest.approx(ex["zvir_half"], rel=1e-6, abs=1e-6)
+    _check_common(case, r, r["RTHL"], r["Fbarx"], rad[ex["inside"]] ** 2)
+    sel, _ = PC.inside_cells(case, r["RTHL"], frame=case.nbuff)
+    assert np.array_equal(mask.astype(bool), sel)
+
+
+def test_pin_d2F_stops_at_the_grid_edge_and_mask_frame():
+    """A peak 6 cells from the x = 0 face with R_TH > 6: d2F averages lapd
+    over cells 2.. in shell order and stops at the first cell outside the
+    grid (quirk Q7); including the centre, or skipping instead of stopping,
+    gives another value.  The mask still holds exactly the cells within
+    R_TH inside the frame."""
+    case = PC.bump_case(centre=(6, 23, 23))
+    t, b = PC.constant_table(1.8)
+    mask = np.zeros(case.delta.shape, np.int8)
+    r, st, _ = _run_case(case, t, b, 1.0, 1.5, mask)
+    assert st == 0 and r["RTHL"] > 7
+    sh = O.Shells(case.rmax)
+    exp = PC.d2F_walk(case, r["RTHL"], sh.di, sh.dj, sh.dk, sh.r2)
+    assert r["d2F"] == pytest.approx(exp, rel=1e-12)
+    sel, q = PC.inside_cells(case, r["RTHL"], exclude_centre=True)
+    assert abs(case.lapd.astype(np.float64)[sel].mean() - exp) > 1e-3   # skipping would differ
+    sel_f, _ = PC.inside_cells(case, r["RTHL"], frame=case.nbuff)
+    assert np.array_equal(mask.astype(bool), sel_f)
