@@ -27,117 +27,201 @@
 
 namespace pp {
 
-constexpr int kStencilBlock = 256;
-
 // ------------------------------------------------------------- window
-__global__ void __launch_bounds__(256) k_window(const float2 *__restrict__ in, float2 *__restrict__ out,
-                                                int n, int nk, long long total, double dk, double Rf,
-                                                int wsmooth, float scale)
+// W depends on |k| only through kx^2 + ky^2 + kz^2 with kx = dk ix,
+// ky = dk fy, kz = dk fz; negating fy or fz is exact, so the four elements
+// (ix, +-fy, +-fz) share one Float64 evaluation bit for bit.  A thread takes
+// one (ix, |fy|, |fz|) and scales its (up to) four mirror elements; the grid
+// is (ix, |fy|) linearised in x and |fz| in y.  With two windows the thread
+// reads each element once and writes both products.
+struct WinSel {
+    float2 *out;
+    int wsmooth;  // PP_WINDOW_*; < 0: no output
+};
+
+__device__ __forceinline__ double window_value(int wsmooth, double k, double fkR)
+{
+    if (wsmooth == PP_WINDOW_GAUSSIAN) {
+        double t = fkR / 2.0;
+        return exp(-(t * t) / 2.0);
+    }
+    if (wsmooth == PP_WINDOW_K2GAUSS) {
+        double t = fkR / 2.0;
+        return exp(-(t * t) / 2.0) * (k * k);
+    }
+    double s, c;
+    sincos(fkR, &s, &c);
+    return 3.0 * (s - fkR * c) / (fkR * fkR * fkR);
+}
+
+__global__ void __launch_bounds__(256) k_window(const float2 *__restrict__ in, WinSel a, WinSel b, int n, int nk,
+                                                double dk, double Rf, float scale)
 {
 #pragma clang fp contract(off)
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        int ix = (int)(idx % nk);
-        long long rem = idx / nk;
-        int iy = (int)(rem % n);
-        int iz = (int)(rem / n);
-        int fy = iy < (n + 1) / 2 ? iy : iy - n;
-        int fz = iz < (n + 1) / 2 ? iz : iz - n;
-        double kx = dk * (double)ix, ky = dk * (double)fy, kz = dk * (double)fz;
-        double k = sqrt(kx * kx + ky * ky + kz * kz);
-        double fkR = k * Rf;
-        float2 v = in[idx];
-        if (fkR != 0.0) {
-            double w;
-            if (wsmooth == PP_WINDOW_GAUSSIAN) {
-                double t = fkR / 2.0;
-                w = exp(-(t * t) / 2.0);
-            } else if (wsmooth == PP_WINDOW_K2GAUSS) {
-                double t = fkR / 2.0;
-                w = exp(-(t * t) / 2.0) * (k * k);
-            } else {
-                w = 3.0 * (sin(fkR) - fkR * cos(fkR)) / (fkR * fkR * fkR);
-            }
-            float wf = (float)w;
-            v.x = v.x * wf;
-            v.y = v.y * wf;
-        }
-        v.x = v.x * scale;
-        v.y = v.y * scale;
-        out[idx] = v;
+    const int half = n / 2;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const int hy = idx / nk;
+    const int ix = idx - hy * nk;
+    const int hz = blockIdx.y;
+    if (hy > half) return;
+    const double kx = dk * (double)ix, ky = dk * (double)hy, kz = dk * (double)hz;
+    const double k = sqrt(kx * kx + ky * ky + kz * kz);
+    const double fkR = k * Rf;
+    float wa = 1.f, wb = 1.f;
+    const bool scaled = fkR != 0.0;
+    if (scaled) {
+        wa = (float)window_value(a.wsmooth, k, fkR);
+        if (b.wsmooth >= 0) wb = (float)window_value(b.wsmooth, k, fkR);
     }
+    const int iy2 = hy == 0 ? 0 : n - hy, iz2 = hz == 0 ? 0 : n - hz;
+    const int ny = iy2 == hy ? 1 : 2, nz = iz2 == hz ? 1 : 2;
+    for (int qz = 0; qz < nz; qz++) {
+        const int iz = qz ? iz2 : hz;
+        for (int qy = 0; qy < ny; qy++) {
+            const int iy = qy ? iy2 : hy;
+            const size_t e = (size_t)ix + (size_t)nk * ((size_t)iy + (size_t)n * (size_t)iz);
+            const float2 v = in[e];
+            float2 r = v;
+            if (scaled) { r.x = r.x * wa; r.y = r.y * wa; }
+            r.x = r.x * scale;
+            r.y = r.y * scale;
+            a.out[e] = r;
+            if (b.wsmooth >= 0) {
+                float2 s = v;
+                if (scaled) { s.x = s.x * wb; s.y = s.y * wb; }
+                s.x = s.x * scale;
+                s.y = s.y * scale;
+                b.out[e] = s;
+            }
+        }
+    }
+}
+
+static int launch_window2(pp_ctx *ctx, const void *in, void *out_a, int ws_a, void *out_b, int ws_b, int n,
+                          double boxsize, double Rf, float scale)
+{
+    const int nk = n / 2 + 1, half = n / 2;
+    const long long per = (long long)nk * (half + 1);
+    PP_ARG(per < (1ll << 31) && half + 1 <= 65535, "mesh %d too large for the window grid", n);
+    const double dk = 2.0 * M_PI / boxsize;
+    dim3 grid((unsigned)((per + 255) / 256), (unsigned)(half + 1));
+    WinSel a{(float2 *)out_a, ws_a}, b{(float2 *)out_b, out_b ? ws_b : -1};
+    hipLaunchKernelGGL(k_window, grid, dim3(256), 0, ctx->stream, (const float2 *)in, a, b, n, nk, dk, Rf, scale);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
 }
 
 static int launch_window(pp_ctx *ctx, const void *in, void *out, int n, double boxsize, double Rf,
                          int wsmooth, float scale)
 {
-    int nk = n / 2 + 1;
-    long long total = (long long)nk * n * n;
-    double dk = 2.0 * M_PI / boxsize;
-    long long blocks = (total + 255) / 256;
-    int grid = (int)std::min<long long>(blocks, (long long)ctx->num_cu * 32);
-    hipLaunchKernelGGL(k_window, dim3(grid), dim3(256), 0, ctx->stream, (const float2 *)in,
-                       (float2 *)out, n, nk, total, dk, Rf, wsmooth, scale);
-    PP_HIP(hipGetLastError());
-    return PP_OK;
+    return launch_window2(ctx, in, out, wsmooth, nullptr, -1, n, boxsize, Rf, scale);
 }
 
 // ------------------------------------------------------------- stencil
+// One pass over the field per filter.  A workgroup takes a 64 x 4 tile of
+// inner (x, y) columns and marches along z through kStencilKZ planes: each
+// plane's (64+2) x (4+2) neighbourhood is staged in LDS, every column keeps
+// the 3x3 (x, y) maxima of planes z-1, z, z+1 in registers, and the strict
+// 27-point test "no neighbour above ff" is ff >= max of the 27 values.  A
+// wave is one x-run of 64 cells of one row, so its ballot of accepted cells
+// is one 64-bit word of the row's peak bit mask.  The mask byte is read only
+// for cells that pass the field test.
+//
+// A second, small pass turns the bit mask into the output: the exclusive
+// scan of the words' popcounts in (k, j, x) order is the CPU loop's order
+// (k outer, j, i inner), so each set bit is written at its rank with no
+// atomics, the first max_peaks ranks are kept (PeakFind.jl:70-72) and only
+// the kept peak cells are masked.
+constexpr int kStencilBlock = 256;
+constexpr int kStencilTY = kStencilBlock / 64;
+constexpr int kStencilKZ = 32;
+
 struct StencilGeom {
     int n1, n2, n3, nbuff;
     int in1, in2, in3;
-    long long total;
+    int words;            // 64-bit words per inner row, ceil(in1 / 64)
+    long long nwords;     // words * in2 * in3
 };
 
-__device__ __forceinline__ bool stencil_pred(const float *__restrict__ d, const int8_t *__restrict__ mask,
-                                             const StencilGeom &g, long long li, float fcrit,
-                                             int &i, int &j, int &k, float &ff)
+static StencilGeom stencil_geom(int n1, int n2, int n3, int nbuff)
 {
-    i = (int)(li % g.in1) + g.nbuff;
-    long long rem = li / g.in1;
-    j = (int)(rem % g.in2) + g.nbuff;
-    k = (int)(rem / g.in2) + g.nbuff;
-    const long long s1 = g.n1, s12 = (long long)g.n1 * g.n2;
-    const long long c = i + j * s1 + k * s12;
-    if (mask[c] == 1) return false;
-    ff = d[c];
-    if (ff < fcrit) return false;
-#pragma unroll
-    for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++) {
-            const float *row = d + c + dy * s1 + dz * s12;
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++)
-                if (ff < row[dx]) return false;
-        }
-    return true;
+    StencilGeom g;
+    g.n1 = n1; g.n2 = n2; g.n3 = n3; g.nbuff = nbuff;
+    g.in1 = std::max(0, n1 - 2 * nbuff);
+    g.in2 = std::max(0, n2 - 2 * nbuff);
+    g.in3 = std::max(0, n3 - 2 * nbuff);
+    g.words = (g.in1 + 63) / 64;
+    g.nwords = (long long)g.words * g.in2 * g.in3;
+    return g;
 }
 
-__global__ void __launch_bounds__(kStencilBlock) k_stencil_count(const float *__restrict__ d,
-                                                                 const int8_t *__restrict__ mask,
-                                                                 StencilGeom g, float fcrit,
-                                                                 int *__restrict__ block_count)
+__global__ void __launch_bounds__(kStencilBlock) k_peak_bits(const float *__restrict__ d,
+                                                             const int8_t *__restrict__ mask, StencilGeom g,
+                                                             float fcrit, uint64_t *__restrict__ bits)
 {
-    long long li = (long long)blockIdx.x * kStencilBlock + threadIdx.x;
-    bool f = false;
-    if (li < g.total) {
-        int i, j, k;
-        float ff;
-        f = stencil_pred(d, mask, g, li, fcrit, i, j, k, ff);
-    }
-    unsigned long long b = __ballot(f);
-    __shared__ int wc[kStencilBlock / 64];
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wc[wid] = __popcll(b);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
+    constexpr int LX = 64 + 2, LY = kStencilTY + 2, LN = LX * LY;
+    __shared__ float tile[2][LN];
+    const int lane = threadIdx.x & 63, wy = threadIdx.x >> 6;
+    const int x0 = g.nbuff + 64 * blockIdx.x, y0 = g.nbuff + kStencilTY * blockIdx.y;
+    const int z0 = g.nbuff + kStencilKZ * blockIdx.z;
+    const int z1 = min(z0 + kStencilKZ, g.nbuff + g.in3);
+    const long long s1 = g.n1, s12 = (long long)g.n1 * g.n2;
+    const int x = x0 + lane, y = y0 + wy;
+    const bool inner = x < g.nbuff + g.in1 && y < g.nbuff + g.in2;
+    // the two elements of the staged plane each thread loads (clamped to
+    // the grid: clamped cells only neighbour columns outside the inner cube)
+    long long off[2];
+    bool has[2];
 #pragma unroll
-        for (int w = 0; w < kStencilBlock / 64; w++) s += wc[w];
-        block_count[blockIdx.x] = s;
+    for (int q = 0; q < 2; q++) {
+        const int e = threadIdx.x + q * kStencilBlock;
+        has[q] = e < LN;
+        const int ly = e / LX, lx = e - ly * LX;
+        const int gx = min(x0 - 1 + lx, g.n1 - 1), gy = min(y0 - 1 + ly, g.n2 - 1);
+        off[q] = gx + gy * s1;
+    }
+    auto stage = [&](int z, int buf) {
+        const float *p = d + (long long)z * s12;
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (has[q]) tile[buf][threadIdx.x + q * kStencilBlock] = p[off[q]];
+    };
+    auto max9 = [&](int buf, float &centre) {
+        const float *t = tile[buf] + wy * LX + lane;
+        centre = t[LX + 1];
+        float m = fmaxf(fmaxf(t[0], t[1]), t[2]);
+        m = fmaxf(m, fmaxf(fmaxf(t[LX], t[LX + 1]), t[LX + 2]));
+        m = fmaxf(m, fmaxf(fmaxf(t[2 * LX], t[2 * LX + 1]), t[2 * LX + 2]));
+        return m;
+    };
+    float c_prev, c_cur, c_next;
+    stage(z0 - 1, 0);
+    __syncthreads();
+    float m_prev = max9(0, c_prev);
+    stage(z0, 1);
+    __syncthreads();
+    float m_cur = max9(1, c_cur);
+    const long long word_row = (long long)g.words * g.in2;
+    for (int z = z0; z < z1; z++) {
+        const int buf = (z - z0) & 1;  // plane z+1 goes where plane z-1 was
+        stage(z + 1, buf);
+        __syncthreads();
+        const float m_next = max9(buf, c_next);
+        const float m27 = fmaxf(fmaxf(m_prev, m_cur), m_next);
+        bool pk = inner && c_cur >= fcrit && !(c_cur < m27);
+        if (pk) pk = mask[x + y * s1 + (long long)z * s12] != 1;
+        const unsigned long long w = __ballot(pk);
+        if (lane == 0 && y < g.nbuff + g.in2)
+            bits[(long long)(z - g.nbuff) * word_row + (long long)(y - g.nbuff) * g.words + blockIdx.x] = w;
+        m_prev = m_cur;
+        m_cur = m_next;
+        c_cur = c_next;
     }
 }
+
+struct Popc {
+    __device__ __forceinline__ int operator()(const uint64_t &w) const { return __popcll(w); }
+};
 
 struct PeakOut {
     int32_t *i, *j, *k, *filt;
@@ -147,43 +231,44 @@ struct PeakOut {
 
 // counters[0] = peaks written so far (all filters), counters[1] = peaks found
 // (capped per filter), counters[2] = first index of the current filter.
-__global__ void __launch_bounds__(kStencilBlock) k_stencil_write(const float *__restrict__ d, int8_t *__restrict__ mask,
-                                                                 StencilGeom g, float fcrit,
-                                                                 const int *__restrict__ block_off,
-                                                                 const long long *__restrict__ counters,
-                                                                 long long max_peaks, int filt, PeakOut o)
+__global__ void __launch_bounds__(256) k_peak_emit(const float *__restrict__ d, int8_t *__restrict__ mask,
+                                                   StencilGeom g, const uint64_t *__restrict__ bits,
+                                                   const int *__restrict__ word_off,
+                                                   const long long *__restrict__ counters, long long max_peaks,
+                                                   int filt, PeakOut o)
 {
-    long long li = (long long)blockIdx.x * kStencilBlock + threadIdx.x;
-    bool f = false;
-    int i = 0, j = 0, k = 0;
-    float ff = 0.f;
-    if (li < g.total) f = stencil_pred(d, mask, g, li, fcrit, i, j, k, ff);
-    unsigned long long b = __ballot(f);
-    __shared__ int wc[kStencilBlock / 64];
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wc[wid] = __popcll(b);
-    __syncthreads();
-    if (!f) return;
-    int before = 0;
-    for (int w = 0; w < wid; w++) before += wc[w];
-    unsigned long long lt = lane == 0 ? 0ull : (b & (~0ull >> (64 - lane)));
-    long long rank = (long long)block_off[blockIdx.x] + before + __popcll(lt);
-    if (rank >= max_peaks) return;
-    const long long c = i + (long long)j * g.n1 + (long long)k * g.n1 * g.n2;
-    mask[c] = 1;
-    long long slot = counters[0] + rank;
-    if (slot >= o.capacity) return;
-    o.i[slot] = i;
-    o.j[slot] = j;
-    o.k[slot] = k;
-    o.filt[slot] = filt;
-    o.delta_s[slot] = ff;
+    const long long w = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (w >= g.nwords) return;
+    uint64_t b = bits[w];
+    if (!b) return;
+    const long long row = w / g.words;
+    const int wx = (int)(w - row * g.words);
+    const int j = g.nbuff + (int)(row % g.in2), k = g.nbuff + (int)(row / g.in2);
+    long long rank = word_off[w];
+    const long long base = counters[0];
+    while (b) {
+        if (rank >= max_peaks) return;
+        const int bit = __ffsll((unsigned long long)b) - 1;
+        b &= b - 1;
+        const int i = g.nbuff + 64 * wx + bit;
+        const long long c = i + (long long)j * g.n1 + (long long)k * g.n1 * g.n2;
+        mask[c] = 1;
+        const long long slot = base + rank;
+        if (slot < o.capacity) {
+            o.i[slot] = i;
+            o.j[slot] = j;
+            o.k[slot] = k;
+            o.filt[slot] = filt;
+            o.delta_s[slot] = d[c];
+        }
+        rank++;
+    }
 }
 
-__global__ void k_stencil_finish(const int *__restrict__ block_off, const int *__restrict__ block_count,
-                                 int nblocks, long long max_peaks, long long *__restrict__ counters)
+__global__ void k_stencil_finish(const uint64_t *__restrict__ bits, const int *__restrict__ word_off,
+                                 long long nwords, long long max_peaks, long long *__restrict__ counters)
 {
-    long long found = nblocks > 0 ? (long long)block_off[nblocks - 1] + block_count[nblocks - 1] : 0;
+    long long found = nwords > 0 ? (long long)word_off[nwords - 1] + __popcll(bits[nwords - 1]) : 0;
     long long kept = found < max_peaks ? found : max_peaks;
     counters[2] = counters[0];
     counters[0] += kept;
@@ -202,38 +287,88 @@ __global__ void k_sample_field(const float *__restrict__ f, int n1, int n2, cons
         out[s] = f[pi[s] + (long long)pj[s] * n1 + (long long)pk[s] * n1 * n2];
 }
 
-// Stencil over one field; peaks appended after counters[0].
+// Device scratch of one stencil pass: the peak bit mask, the words' output
+// offsets and the scan's temporary storage, carved from one buffer.
+struct StencilScratch {
+    uint64_t *bits;
+    int *off;
+    void *tmp;
+    size_t tmp_bytes;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int stencil_scan_bytes(const StencilGeom &g, size_t *tmp)
+{
+    hipcub::TransformInputIterator<int, Popc, const uint64_t *> it((const uint64_t *)nullptr, Popc());
+    *tmp = 0;
+    PP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, *tmp, it, (int *)nullptr, (int)std::max(1ll, g.nwords)));
+    return PP_OK;
+}
+
+static int stencil_scratch_bytes(const StencilGeom &g, size_t *bytes)
+{
+    size_t tmp = 0;
+    PP_CHECK(stencil_scan_bytes(g, &tmp));
+    *bytes = align256((size_t)g.nwords * 8) + align256((size_t)g.nwords * 4) + align256(tmp);
+    return PP_OK;
+}
+
+static StencilScratch stencil_carve(void *base, const StencilGeom &g, size_t tmp)
+{
+    StencilScratch s;
+    char *p = (char *)base;
+    s.bits = (uint64_t *)p;
+    p += align256((size_t)g.nwords * 8);
+    s.off = (int *)p;
+    p += align256((size_t)g.nwords * 4);
+    s.tmp = p;
+    s.tmp_bytes = tmp;
+    return s;
+}
+
+// Stencil over one field with the scratch given; peaks appended after
+// counters[0].
+static int run_stencil_with(pp_ctx *ctx, const float *d_field, int8_t *d_mask, const StencilGeom &g,
+                            float fcrit, long long max_peaks, int filt, const PeakOut &o, long long *d_counters,
+                            const StencilScratch &s)
+{
+    PP_ARG(g.nwords < (1ll << 31), "inner cube too large for the stencil");
+    if (g.nwords == 0) {
+        hipLaunchKernelGGL(k_stencil_finish, dim3(1), dim3(1), 0, ctx->stream, (const uint64_t *)nullptr,
+                           (const int *)nullptr, 0ll, max_peaks, d_counters);
+        PP_HIP(hipGetLastError());
+        return PP_OK;
+    }
+    dim3 grid(g.words, (g.in2 + kStencilTY - 1) / kStencilTY, (g.in3 + kStencilKZ - 1) / kStencilKZ);
+    PP_ARG(grid.y <= 65535 && grid.z <= 65535, "inner cube too large for the stencil grid");
+    hipLaunchKernelGGL(k_peak_bits, grid, dim3(kStencilBlock), 0, ctx->stream, d_field, (const int8_t *)d_mask, g,
+                       fcrit, s.bits);
+    PP_HIP(hipGetLastError());
+    hipcub::TransformInputIterator<int, Popc, const uint64_t *> it(s.bits, Popc());
+    size_t tmp = s.tmp_bytes;
+    PP_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp, tmp, it, s.off, (int)g.nwords, ctx->stream));
+    const unsigned eblocks = (unsigned)((g.nwords + 255) / 256);
+    hipLaunchKernelGGL(k_peak_emit, dim3(eblocks), dim3(256), 0, ctx->stream, d_field, d_mask, g,
+                       (const uint64_t *)s.bits, (const int *)s.off, (const long long *)d_counters, max_peaks,
+                       filt, o);
+    PP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_stencil_finish, dim3(1), dim3(1), 0, ctx->stream, (const uint64_t *)s.bits,
+                       (const int *)s.off, g.nwords, max_peaks, d_counters);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}
+
 static int run_stencil(pp_ctx *ctx, const float *d_field, int8_t *d_mask, const StencilGeom &g,
                        float fcrit, long long max_peaks, int filt, const PeakOut &o,
                        long long *d_counters)
 {
-    long long nblocks_ll = (g.total + kStencilBlock - 1) / kStencilBlock;
-    PP_ARG(nblocks_ll < (1ll << 31), "inner cube too large for the stencil grid");
-    int nblocks = (int)nblocks_ll;
-    if (nblocks == 0) {
-        hipLaunchKernelGGL(k_stencil_finish, dim3(1), dim3(1), 0, ctx->stream, (const int *)nullptr,
-                           (const int *)nullptr, 0, max_peaks, d_counters);
-        PP_HIP(hipGetLastError());
-        return PP_OK;
-    }
-    PP_CHECK(ctx->d_scan.grow(2 * (size_t)nblocks * sizeof(int)));
-    int *d_cnt = ctx->d_scan.as<int>();
-    int *d_off = d_cnt + nblocks;
-    hipLaunchKernelGGL(k_stencil_count, dim3(nblocks), dim3(kStencilBlock), 0, ctx->stream, d_field,
-                       (const int8_t *)d_mask, g, fcrit, d_cnt);
-    PP_HIP(hipGetLastError());
-    size_t tmp = 0;
-    PP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_cnt, d_off, nblocks, ctx->stream));
-    PP_CHECK(ctx->d_cub.grow(tmp));
-    PP_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_cub.ptr, tmp, d_cnt, d_off, nblocks, ctx->stream));
-    hipLaunchKernelGGL(k_stencil_write, dim3(nblocks), dim3(kStencilBlock), 0, ctx->stream, d_field,
-                       d_mask, g, fcrit, (const int *)d_off, (const long long *)d_counters, max_peaks,
-                       filt, o);
-    PP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_stencil_finish, dim3(1), dim3(1), 0, ctx->stream, (const int *)d_off,
-                       (const int *)d_cnt, nblocks, max_peaks, d_counters);
-    PP_HIP(hipGetLastError());
-    return PP_OK;
+    size_t bytes = 0, tmp = 0;
+    PP_CHECK(stencil_scan_bytes(g, &tmp));
+    PP_CHECK(stencil_scratch_bytes(g, &bytes));
+    PP_CHECK(ctx->d_scan.grow(bytes));
+    return run_stencil_with(ctx, d_field, d_mask, g, fcrit, max_peaks, filt, o, d_counters,
+                            stencil_carve(ctx->d_scan.ptr, g, tmp));
 }
 
 static int alloc_peak_out(pp_ctx *ctx, long long cap, PeakOut &o, float **lapd, long long **counters)
@@ -368,12 +503,7 @@ int pp_find_peaks_field(pp_ctx *ctx, const float *delta_s, int8_t *mask, int on_
     PP_ARG(out->capacity >= 0, "negative capacity");
     PP_CHECK(ensure_device(ctx));
     if (max_peaks <= 0) max_peaks = 2000000;
-    StencilGeom g;
-    g.n1 = n1; g.n2 = n2; g.n3 = n3; g.nbuff = nbuff;
-    g.in1 = std::max(0, n1 - 2 * nbuff);
-    g.in2 = std::max(0, n2 - 2 * nbuff);
-    g.in3 = std::max(0, n3 - 2 * nbuff);
-    g.total = (long long)g.in1 * g.in2 * g.in3;
+    const StencilGeom g = stencil_geom(n1, n2, n3, nbuff);
     size_t N = (size_t)n1 * n2 * n3;
     const float *d_f = delta_s;
     int8_t *d_m = mask;
@@ -417,7 +547,10 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
     PP_CHECK(ctx->d_k0.grow(cb));
     PP_CHECK(ctx->d_k1.grow(cb));
     PP_CHECK(ctx->d_field_a.grow(rb));
-    if (want_lapd) PP_CHECK(ctx->d_field_b.grow(rb));
+    if (want_lapd) {
+        PP_CHECK(ctx->d_k2.grow(cb));
+        PP_CHECK(ctx->d_field_b.grow(rb));
+    }
     PP_CHECK(ctx->d_mask.grow(N));
     PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     const float *d_delta = delta;
@@ -438,21 +571,19 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
     float *d_lapd;
     long long *d_counters;
     PP_CHECK(alloc_peak_out(ctx, out->capacity, o, &d_lapd, &d_counters));
-    StencilGeom g;
-    g.n1 = g.n2 = g.n3 = n;
-    g.nbuff = nbuff;
-    g.in1 = g.in2 = g.in3 = std::max(0, n - 2 * nbuff);
-    g.total = (long long)g.in1 * g.in2 * g.in3;
+    const StencilGeom g = stencil_geom(n, n, n, nbuff);
     const float inv = (float)(1.0 / ((double)n * n * n));
     for (int f = 0; f < nfilt; f++) {
-        PP_CHECK(launch_window(ctx, ctx->d_k0.ptr, ctx->d_k1.ptr, n, boxsize, Rf[f], wsmooth, inv));
+        // one read of delta_k gives the filter's spectrum and, with want_lapd,
+        // the wsmooth = 3 spectrum sampled at the filter's new peaks
+        PP_CHECK(launch_window2(ctx, ctx->d_k0.ptr, ctx->d_k1.ptr, wsmooth, want_lapd ? ctx->d_k2.ptr : nullptr,
+                                PP_WINDOW_K2GAUSS, n, boxsize, Rf[f], inv));
         void *ib[1] = {ctx->d_k1.ptr}, *ob[1] = {ctx->d_field_a.ptr};
         PP_FFT(rocfft_execute(ctx->fft.c2r, ib, ob, ctx->fft.info));
         PP_CHECK(run_stencil(ctx, ctx->d_field_a.as<float>(), d_mask, g, fcrit[f], max_peaks_per_filter, f,
                              o, d_counters));
         if (want_lapd) {
-            PP_CHECK(launch_window(ctx, ctx->d_k0.ptr, ctx->d_k1.ptr, n, boxsize, Rf[f], PP_WINDOW_K2GAUSS, inv));
-            void *ib2[1] = {ctx->d_k1.ptr}, *ob2[1] = {ctx->d_field_b.ptr};
+            void *ib2[1] = {ctx->d_k2.ptr}, *ob2[1] = {ctx->d_field_b.ptr};
             PP_FFT(rocfft_execute(ctx->fft.c2r, ib2, ob2, ctx->fft.info));
             hipLaunchKernelGGL(k_sample_field, dim3(ctx->num_cu * 4), dim3(256), 0, ctx->stream,
                                ctx->d_field_b.as<float>(), n, n, (const int32_t *)o.i, (const int32_t *)o.j,
@@ -472,7 +603,7 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
 }  // extern "C"
 
 // ------------------------------------------------------------ input noise
-// fill_noise_threefry! (src/InitialConditions/RandomField.jl:269-306): cell
+// fill_noise_threefry! (src/InitialConditions/RandomField.jl:46-81): cell
 // idx (1-based, x fastest) takes the first normal of counter pair
 // (idx-1) >> 1 when idx-1 is even, the second otherwise; each pair is
 // Threefry-2x64-20 of counter (pair, 0) under key (seed, 0) (Salmon et al.
