@@ -10,6 +10,7 @@ Each wrapper mirrors one reference function (file:line cited in the C source).
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -313,3 +314,50 @@ def analyse_peaks(grid: Grid, ipp, ir2min, ZZon, Rf, fcrit, alatt, table: Table,
                            np.ascontiguousarray(fcrit, np.float64), alatt, C.byref(table.c),
                            C.byref(shells.c), nbuff, rmax2rs, nthreads, res.ctypes.data, st)
     return res, st
+
+
+# ---- ellipsoidal collapse (EllipsoidalCollapse.jl RK4 path; CollapseTable.jl) ----
+_lib.orc_elliptic_rd.restype = C.c_double
+_lib.orc_elliptic_rd.argtypes = [C.c_double] * 3
+_lib.orc_get_b_2.argtypes = [C.c_double, C.c_double, C.c_double, _f64p]
+_lib.orc_evolve_ellipse.restype = C.c_double
+_lib.orc_evolve_ellipse.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_int)]
+_lib.orc_dfnofa.restype = C.c_double
+_lib.orc_dfnofa.argtypes = [C.c_void_p, C.c_double]
+_lib.orc_make_collapse_table.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int] + \
+    [C.c_double] * 6 + [C.c_void_p]
+
+TABLE_BOUNDS = (math.log10(1.5), math.log10(8.0), 0.0, 0.5, -1.0 + 1e-4, 1.0 - 1e-4)
+
+
+def elliptic_rd(x, y, z):
+    return _lib.orc_elliptic_rd(x, y, z)
+
+
+def get_b_2(a1, a2, a3):
+    b = np.zeros(3)
+    _lib.orc_get_b_2(a1, a2, a3, b)
+    return b
+
+
+class Ellipsoid:
+    """evolve_ellipse_full with the Fortran-compatible RK4 solver and the
+    reference's default parameters, for one cosmology."""
+
+    def __init__(self, Om=0.315, OB=0.049, OL=0.685, h=0.674):
+        self.Om, self.OL = Om, OL
+        self.dlin = Dlinear(Om, OB, OL, h)
+
+    def zvir(self, Frho, e_v, p_v):
+        n = C.c_int()
+        z = _lib.orc_evolve_ellipse(self.dlin.buf, self.Om, self.OL, Frho, e_v, p_v, C.byref(n))
+        return z, n.value
+
+    def dfnofa(self, a):
+        return _lib.orc_dfnofa(self.dlin.buf, a)
+
+    def table(self, nx=50, ny=20, nz=20, bounds=TABLE_BOUNDS):
+        out = np.zeros(nx * ny * nz, np.float32)
+        _lib.orc_make_collapse_table(self.dlin.buf, self.Om, self.OL, nx, ny, nz, *bounds, out.ctypes.data)
+        return out.reshape(nz, ny, nx).transpose(2, 1, 0)   # [ix, iy, iz]

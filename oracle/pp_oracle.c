@@ -1094,3 +1094,181 @@ int orc_analyse_peaks(const orc_grid *pg, int64_t npk, const int64_t *ipp,
 }
 
 size_t orc_result_size(void) { return sizeof(orc_result); }
+
+/* ------------------------------------------------------------------------ */
+/* Ellipsoidal collapse, Fortran-compatible RK4 path                        */
+/* (src/EllipsoidalCollapse/EllipsoidalCollapse.jl:60-361, 510-624) and the  */
+/* table grid (src/EllipsoidalCollapse/CollapseTable.jl:9-45).               */
+/* Parameters fixed at the reference's defaults: iforce_strat 4, ivir_strat */
+/* 2, fcoll (0.01, 0.171, 0.171), tfac 0.01, nstepmax 10000, zinit_fac 20,  */
+/* iwant_rd 1.                                                               */
+/* ------------------------------------------------------------------------ */
+double orc_dfnofa(const orc_dlin *T, double ai)
+{   /* Dfnofa (Cosmology.jl:296-316): the D column of the same 4-point rule */
+    double Da;
+    (void)orc_dlinear_ab(T, ai, NULL, &Da);
+    return ai * Da;
+}
+
+double orc_elliptic_rd(double x, double y, double z)
+{   /* _elliptic_rd (EllipsoidalCollapse.jl:63-107), Carlson's duplication */
+    const double ERRTOL = 0.05, C1 = 3.0 / 14.0, C2 = 1.0 / 6.0, C3 = 9.0 / 22.0, C4 = 3.0 / 26.0;
+    const double C5 = 0.25 * C3, C6 = 1.5 * C4;
+    double xt = x, yt = y, zt = z, sum = 0.0, fac = 1.0, delx = 0, dely = 0, delz = 0, ave = 0;
+    for (;;) {
+        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
+        double alamb = sx * (sy + sz) + sy * sz;
+        sum += fac / (sz * (zt + alamb));
+        fac *= 0.25;
+        xt = 0.25 * (xt + alamb);
+        yt = 0.25 * (yt + alamb);
+        zt = 0.25 * (zt + alamb);
+        ave = 0.2 * (xt + yt + 3.0 * zt);
+        delx = (ave - xt) / ave;
+        dely = (ave - yt) / ave;
+        delz = (ave - zt) / ave;
+        double mx = fabs(delx);
+        if (fabs(dely) > mx) mx = fabs(dely);
+        if (fabs(delz) > mx) mx = fabs(delz);
+        if (mx <= ERRTOL) break;
+    }
+    double ea = delx * dely, eb = delz * delz, ec = ea - eb, ed = ea - 6.0 * eb, ee = ed + ec + ec;
+    return 3.0 * sum + fac * (1.0 + ed * (-C1 + C5 * ed - C6 * delz * ee) +
+                              delz * (C2 * ee + delz * (-C3 * ec + delz * C4 * ea))) / (ave * sqrt(ave));
+}
+
+void orc_get_b_2(double a1, double a2, double a3, double *b)
+{   /* get_b_2 (EllipsoidalCollapse.jl:113-146), iwant_rd = 1 */
+    const double third = 1.0 / 3.0;
+    double r2 = a2 / a1, r3 = a3 / a1;
+    if (r3 >= r2) {
+        double b3;
+        if (r3 >= 0.999) { b[0] = b[1] = b[2] = third; return; }
+        if (r3 <= 0.001) {
+            double ecc2 = 1.0 - r3 * r3, one_ecc2 = r3 * r3, one_ecc = 0.5 * r3 * r3, ecc = sqrt(ecc2);
+            b3 = 0.5 * (one_ecc2 / ecc2 * (1.0 / one_ecc2 - log((1.0 + ecc) / one_ecc) * 0.5 / ecc));
+        } else {
+            double ecc2 = 1.0 - r3 * r3, ecc = sqrt(ecc2);
+            b3 = 0.5 * ((1.0 - ecc2) / ecc2 * (1.0 / (1.0 - ecc2) - log((1.0 + ecc) / (1.0 - ecc)) * 0.5 / ecc));
+        }
+        b[0] = 1.0 - 2.0 * b3; b[1] = b3; b[2] = b3;
+        return;
+    }
+    double r22 = r2 * r2, r32 = r3 * r3, q = third * r2 * r3;
+    double b3 = q * orc_elliptic_rd(r22, 1.0, r32);
+    double b2 = q * orc_elliptic_rd(r32, 1.0, r22);
+    b[0] = 1.0 - (b2 + b3); b[1] = b2; b[2] = b3;
+}
+
+typedef struct {
+    const orc_dlin *T;
+    double Rvac, Rcurv;             /* Omvac / Omnr, Omcurv / Omnr */
+    int lvirv[3];
+    double a_eq[3], a_3eq2, a_3eq1; /* frozen axis values; thresholds of axes 2 and 1 */
+    double Ha;                      /* last Ha_b_nr */
+    double aLam[3], Frho_3;
+} orc_ell;
+
+static double ha_b_nr(double a, const orc_ell *s)
+{
+    return sqrt((1.0 + s->Rvac * a * a * a + s->Rcurv * a) / a);
+}
+
+static void ell_derivs(double t, double *y, double *dy, orc_ell *s)
+{   /* get_derivs! (EllipsoidalCollapse.jl:213-361), iforce 4, ivir 2; y[0..5] */
+    const double fcoll[3] = {0.01, 0.171, 0.171};
+    for (int i = 0; i < 6; i++) dy[i] = 0.0;
+    double a_b = t, a_b3 = a_b * a_b * a_b;
+    double Ha = ha_b_nr(a_b, s);
+    s->Ha = Ha;
+    double Hi = 1.0 / Ha;
+    if (s->lvirv[2] == 0 && y[2] <= fcoll[2] * a_b) { s->lvirv[2] = 1; s->a_eq[2] = fcoll[2] * a_b; }
+    if (s->lvirv[1] == 0) s->a_3eq2 = fcoll[1] * a_b;
+    if (s->lvirv[0] == 0) s->a_3eq1 = fcoll[0] * a_b;
+    if (s->lvirv[2] == 1 && y[1] <= s->a_3eq2 && s->lvirv[1] == 0) { s->lvirv[1] = 1; s->a_eq[1] = s->a_3eq2; }
+    if (s->lvirv[2] == 1 && y[0] <= s->a_3eq1 && s->lvirv[0] == 0) { s->lvirv[0] = 1; s->a_eq[0] = s->a_3eq1; }
+    for (int ax = 2; ax >= 0; ax--)
+        if (s->lvirv[ax] == 1) { y[ax] = s->a_eq[ax]; y[ax + 3] = 0.0; dy[ax] = 0.0; dy[ax + 3] = 0.0; }
+    double a1 = y[0], a2 = y[1], a3 = y[2];
+    double delta1_e = a_b3 / (a1 * a2 * a3);
+    double d0 = (0.5 / a_b3 - s->Rvac) * Hi;
+    double d1_int = 1.5 * (delta1_e - 1.0) / a_b3 * Hi;
+    double dlin = orc_dlinear_ab(s->T, a_b, NULL, NULL);
+    double d1_ext = 1.5 / a_b3 * Hi * dlin;
+    double b[3];
+    orc_get_b_2(a1, a2, a3, b);
+    for (int ax = 2; ax >= 0; ax--)
+        if (s->lvirv[ax] == 0) {
+            dy[ax] = y[ax + 3] * Hi;
+            dy[ax + 3] = -y[ax] * (d1_int * b[ax] + d0 + d1_ext * (s->aLam[ax] - s->Frho_3));
+        }
+}
+
+static void ell_rk4(double *y, const double *dy, double t, double h, double *yout, orc_ell *s)
+{   /* rk4_homel! (EllipsoidalCollapse.jl:151-178) */
+    double yt[6], dyt[6], dym[6], hh = h * 0.5, h6 = h / 6.0, xh = t + hh;
+    for (int i = 0; i < 6; i++) yt[i] = y[i] + hh * dy[i];
+    ell_derivs(xh, yt, dyt, s);
+    for (int i = 0; i < 6; i++) yt[i] = y[i] + hh * dyt[i];
+    ell_derivs(xh, yt, dym, s);
+    for (int i = 0; i < 6; i++) { yt[i] = y[i] + h * dym[i]; dym[i] = dyt[i] + dym[i]; }
+    ell_derivs(t + h, yt, dyt, s);
+    for (int i = 0; i < 6; i++) yout[i] = y[i] + h6 * (dy[i] + dyt[i] + 2.0 * dym[i]);
+}
+
+/* evolve_ellipse_full, solver :rk4, idynax = 1 (EllipsoidalCollapse.jl:527-624):
+ * returns zvir = 1/a at the first axis's virialisation (1 + z), or -1. */
+double orc_evolve_ellipse(const orc_dlin *T, double Om, double OL, double Frho, double e_v, double p_v,
+                          int *nsteps)
+{
+    orc_ell s;
+    memset(&s, 0, sizeof s);
+    s.T = T;
+    s.Rvac = OL / Om;
+    s.Rcurv = (1.0 - Om - OL) / Om;
+    s.aLam[2] = Frho / 3.0 * (1.0 + 3.0 * e_v + p_v);
+    s.aLam[1] = Frho / 3.0 * (1.0 - 2.0 * p_v);
+    s.aLam[0] = Frho / 3.0 * (1.0 - 3.0 * e_v + p_v);
+    s.Frho_3 = Frho / 3.0;
+    double zinit = 20.0 * Frho, a_b = 1.0 / zinit, t = a_b;
+    double HD_Ha, D_a;
+    double dlin = orc_dlinear_ab(T, a_b, &HD_Ha, &D_a);
+    double Ha = ha_b_nr(a_b, &s);
+    s.Ha = Ha;
+    double y[6], dy[6], yout[6];
+    for (int ax = 0; ax < 3; ax++) {
+        y[ax] = a_b * (1.0 - dlin * s.aLam[ax]);
+        y[ax + 3] = Ha * (1.0 - (1.0 + HD_Ha) * dlin * s.aLam[ax]);
+    }
+    double zdyn1 = -1.0;
+    int ldyn1 = 0, istep = 0;
+    for (;;) {
+        istep++;
+        double a3 = y[2];
+        double dt = 0.01 * a3 * sqrt(a3) * s.Ha;
+        ell_derivs(t, y, dy, &s);
+        ell_rk4(y, dy, t, dt, yout, &s);
+        memcpy(y, yout, sizeof y);
+        t += dt;
+        if (s.lvirv[0] != ldyn1) { zdyn1 = 1.0 / t; ldyn1 = 1; }
+        if (s.lvirv[0] == 1 || istep >= 10000 || t >= 2.0) break;
+    }
+    if (nsteps) *nsteps = istep;
+    return zdyn1;
+}
+
+/* make_table (CollapseTable.jl:33-55): table[i + nx (j + ny k)] = Float32(zvir)
+ * at log10 Frho = X1 + i dX, e = Y1 + j dY, p/e = Z1 + k dZ. */
+void orc_make_collapse_table(const orc_dlin *T, double Om, double OL, int nx, int ny, int nz, double X1, double X2,
+                             double Y1, double Y2, double Z1, double Z2, float *out)
+{
+    double dX = (X2 - X1) / (nx - 1), dY = (Y2 - Y1) / (ny - 1), dZ = (Z2 - Z1) / (nz - 1);
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double x = X1 + i * dX, yv = Y1 + j * dY, z = Z1 + k * dZ;
+                double Frho = pow(10.0, x);
+                out[i + (size_t)nx * (j + (size_t)ny * k)] = (float)orc_evolve_ellipse(T, Om, OL, Frho, yv, z * yv, NULL);
+            }
+}
