@@ -92,6 +92,27 @@ int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 int pp_upload_collapse_table(pp_ctx *ctx, const float *table, int nx, int ny, int nz,
                              double X1, double X2, double Y1, double Y2, double Z1, double Z2);
 
+/* Linear growth tables of a cosmology (DlinearTables, src/Cosmology/
+ * Cosmology.jl:56-66, 148-270): ntab rows of scale factor (ascending),
+ * D(a)/a normalised to D(1) = 1, and dlnD/dlna.  Host arrays. */
+typedef struct {
+    int ntab;
+    const double *atab, *D_a, *dlnD_dlna;
+} pp_growth;
+
+/* Collapse-table generation: make_table (src/EllipsoidalCollapse/
+ * CollapseTable.jl:33-55) with evolve_ellipse_full's Fortran-compatible RK4
+ * solver and default EllipsoidParams (src/EllipsoidalCollapse/
+ * EllipsoidalCollapse.jl:527-624; bin/generate_table.jl --solver rk4), one
+ * table point per GPU lane.  Point (i, j, k) is log10 F = X1 + i dX,
+ * e = Y1 + j dY, p/e = Z1 + k dZ; its value is 1 + z of the first axis's
+ * virialisation, or -1.  table_out (host, nx*ny*nz Float32, x fastest) may
+ * be NULL; install = 1 also makes it the context's collapse table with
+ * these bounds (as pp_upload_collapse_table). */
+int pp_make_collapse_table(pp_ctx *ctx, double Om, double OL, const pp_growth *growth, int nx, int ny,
+                           int nz, double X1, double X2, double Y1, double Y2, double Z1, double Z2,
+                           float *table_out, int install);
+
 /* precompute_shells(rmax) (src/HaloFinder/RadialShell.jl:160-175) grouped per
  * distinct r^2 as build_shell_tables (src/ShellAnalysisGPU.jl:60-88). */
 int pp_build_shell_tables(pp_ctx *ctx, int rmax);
@@ -265,6 +286,56 @@ typedef struct {
 int pp_pack_halos(pp_ctx *ctx, const pp_results *res, const pp_halo_inputs *in, double alatt, double z_max,
                   int extended, float *records, int64_t capacity, int records_on_device, int64_t *nhalo,
                   float *RTHLmax);
+
+/* ------------------------------------------------------------- serving */
+/* A tile server on one GPU: the per-device task loop of run_multitile_split
+ * (src/MultiResolution.jl:1183-1203, one process_tile! per tile, :719-1180)
+ * with continuous batching.  `slots` worker threads each own a pp_ctx on
+ * `device`; a submitted tile waits (FIFO) until a slot is idle, the declared
+ * footprints of the tiles in flight plus its own fit the admission budget and
+ * the device's live free memory covers its footprint, then the slot's thread
+ * calls fn(user, slot_ctx, slot, ticket, request).  fn does the tile's work
+ * on that context and returns PP_OK or a PP_ERR_* code; a failing tile fails
+ * alone.  With device = -1 the slots get no context (ctx = NULL): the
+ * scheduling runs without a GPU. */
+typedef struct pp_server pp_server;
+typedef int (*pp_tile_fn)(void *user, pp_ctx *ctx, int slot, int64_t ticket, void *request);
+typedef struct {
+    int device;           /* GPU of the slots' contexts, or -1             */
+    int slots;            /* 1..64                                         */
+    int queue_capacity;   /* waiting tiles beyond which submit is refused  */
+    size_t budget_bytes;  /* admission budget; 0: hbm_fraction x the free
+                             device memory when the server starts          */
+    double hbm_fraction;
+    pp_tile_fn fn;
+    void *user;
+} pp_serve_config;
+
+int pp_serve_create(const pp_serve_config *cfg, pp_server **out);
+/* Queue a tile.  PP_ERR_BUSY: queue_capacity tiles already wait;
+ * PP_ERR_CAPACITY: footprint_bytes exceeds the budget (it could never be
+ * admitted); PP_ERR_STATE: the server is closing. */
+int pp_serve_submit(pp_server *srv, void *request, size_t footprint_bytes, int64_t *ticket);
+/* Wait up to timeout_ms (< 0: no limit) for the tile to finish: *status is
+ * fn's return (or PP_ERR_INJECTED); times (may be NULL) receives seconds
+ * since the server started at submission, start and completion, and the
+ * slot.  PP_ERR_TIMEOUT if it has not finished in time. */
+int pp_serve_poll(pp_server *srv, int64_t ticket, int timeout_ms, int *status, double *times);
+/* The next finished tile in completion order (each reported once), with
+ * the request pointer it was submitted with (request may be NULL). */
+int pp_serve_next(pp_server *srv, int timeout_ms, int64_t *ticket, void **request, int *status, double *times);
+/* pp_last_error() of a failed tile, as its worker thread left it. */
+int pp_serve_message(pp_server *srv, int64_t ticket, char *buf, size_t len);
+/* Fault injection for failure tests: the tile, not yet started, fails with
+ * PP_ERR_INJECTED when its turn comes instead of running fn. */
+int pp_serve_inject_fault(pp_server *srv, int64_t ticket);
+/* stats[0..6]: waiting, in flight, most in flight so far, finished, failed,
+ * reserved bytes, budget bytes. */
+int pp_serve_stats(pp_server *srv, int64_t *stats);
+/* The context of a slot (NULL with device = -1), e.g. to load tables. */
+int pp_serve_context(pp_server *srv, int slot, pp_ctx **ctx);
+/* Finish every queued tile, stop the workers, destroy the contexts. */
+int pp_serve_destroy(pp_server *srv);
 
 /* Per-stage device time of the last persistent-form call, milliseconds
  * (HIP events), ms[0 .. PP_NSTAGES): ms[0] preparation (from the fork of

@@ -59,7 +59,20 @@ class HaloInputs(C.Structure):
                 ("FcRf", C.c_void_p), ("d2Rf", C.c_void_p), ("on_device", C.c_int)]
 
 
+class Growth(C.Structure):
+    _fields_ = [("ntab", C.c_int), ("atab", C.c_void_p), ("D_a", C.c_void_p), ("dlnD_dlna", C.c_void_p)]
+
+
 _vp, _i, _d, _f, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int64
+
+# pp_tile_fn(user, ctx, slot, ticket, request) -> status
+TILE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_void_p)
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("slots", C.c_int), ("queue_capacity", C.c_int),
+                ("budget_bytes", C.c_size_t), ("hbm_fraction", C.c_double), ("fn", TILE_FN), ("user", C.c_void_p)]
+
 
 SIGNATURES = {
     "pp_last_error": (C.c_char_p, []),
@@ -73,6 +86,7 @@ SIGNATURES = {
     "pp_ctx_wait_stream": (_i, [_vp, _vp]),
     "pp_ctx_signal_stream": (_i, [_vp, _vp]),
     "pp_upload_collapse_table": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d]),
+    "pp_make_collapse_table": (_i, [_vp, _d, _d, C.POINTER(Growth), _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _i]),
     "pp_build_shell_tables": (_i, [_vp, _i]),
     "pp_shell_table_info": (_i, [_vp, C.POINTER(_i64), C.POINTER(C.c_int32)]),
     "pp_window_multiply": (_i, [_vp, _vp, _vp, _i, _i, _d, _d, _i, _f]),
@@ -88,6 +102,15 @@ SIGNATURES = {
     "pp_shell_tunables": (_i, [_vp, _vp, _vp]),
     "pp_last_shell_stages": (_i, [_vp, C.POINTER(_d)]),
     "pp_fill_noise_threefry": (_i, [_vp, _vp, _i64, C.c_uint64]),
+    "pp_serve_create": (_i, [C.POINTER(ServeConfig), C.POINTER(_vp)]),
+    "pp_serve_submit": (_i, [_vp, _vp, C.c_size_t, C.POINTER(_i64)]),
+    "pp_serve_poll": (_i, [_vp, _i64, _i, C.POINTER(_i), C.POINTER(_d)]),
+    "pp_serve_next": (_i, [_vp, _i, C.POINTER(_i64), C.POINTER(_vp), C.POINTER(_i), C.POINTER(_d)]),
+    "pp_serve_message": (_i, [_vp, _i64, C.c_char_p, C.c_size_t]),
+    "pp_serve_inject_fault": (_i, [_vp, _i64]),
+    "pp_serve_stats": (_i, [_vp, C.POINTER(_i64)]),
+    "pp_serve_context": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "pp_serve_destroy": (_i, [_vp]),
     "pp_pack_halos": (_i, [_vp, C.POINTER(Results), C.POINTER(HaloInputs), _d, _d, _i, _vp, _i64, _i,
                            C.POINTER(_i64), C.POINTER(_f)]),
 }

@@ -64,10 +64,24 @@ class Context:
         self.h = h
         self.device = device
         self.rmax = None
+        self._owned = True
+
+    @classmethod
+    def from_handle(cls, handle: int, device: int = 0) -> "Context":
+        """A non-owning wrapper of an existing pp_ctx (e.g. a serving slot's
+        context); close() leaves the context to its owner."""
+        self = cls.__new__(cls)
+        self.lib = L.load()
+        self.h = C.c_void_p(handle)
+        self.device = device
+        self.rmax = None
+        self._owned = False
+        return self
 
     def close(self):
         if self.h:
-            self.lib.pp_ctx_destroy(self.h)
+            if self._owned:
+                self.lib.pp_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -129,7 +143,7 @@ class Context:
 
     def fill_noise(self, n: int, seed: int, out=None):
         """Threefry white noise of an n^3 tile as a CUDA float32 tensor
-        (RandomField.jl:269-306), generated on the context's stream and
+        (RandomField.jl:46-81), generated on the context's stream and
         ordered before later work on torch's current stream."""
         import torch
         if out is None:

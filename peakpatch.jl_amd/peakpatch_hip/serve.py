@@ -105,71 +105,118 @@ def poisson_stream(ntiles: int, rate: float, seed: int = 0, nmesh=SERVE_NMESH, w
     return out
 
 
-class AdmissionError(RuntimeError):
-    pass
-
-
-@dataclass
-class _Slot:
-    index: int
-    state: object = None           # what run_tile needs (Context, stream, ...)
-    busy: bool = False
-
-
 class TileServer:
-    """Continuous-batching tile server.  ``run_tile(slot_state, request)``
-    returns a TileResult; by default the GPU pipeline of ``GpuTileRunner``."""
+    """Continuous-batching tile server over the library's serving loop
+    (pp_serve_*): scheduling, admission and the slot threads are native;
+    ``run_tile(slot_state, request)`` returns a TileResult.
+
+    Without ``run_tile`` the server runs the GPU pipeline of
+    ``GpuTileRunner`` on the contexts the native server creates for its slots
+    on ``device``, and admits by device memory: the budget is
+    ``hbm_fraction`` of the free memory when the server starts (or
+    ``budget_bytes``), and a tile also needs the live free memory to cover
+    its footprint.  With ``run_tile`` the slots have no native context (the
+    caller's ``slot_states`` carry whatever the pipeline needs) and the
+    budget is ``budget_bytes`` (unbounded if None)."""
 
     def __init__(self, slots: int = 4, budget_bytes: int | None = None, run_tile=None, slot_states=None,
-                 footprint=footprint_bytes, device: int = 0, hbm_fraction: float = 0.8):
+                 footprint=footprint_bytes, device: int = 0, hbm_fraction: float = 0.8, runner=None,
+                 queue_capacity: int = 1 << 20):
+        import ctypes as C
+
+        from . import _lib as L
+        self._L = L
+        self._lib = L.load()
         self._footprint = footprint
-        if run_tile is None:
-            runner = GpuTileRunner(device)
-            run_tile = runner.run
-            slot_states = [runner.make_slot() for _ in range(slots)]
-            if budget_bytes is None:
-                free, _ = slot_states[0]["ctx"].mem_info()
-                budget_bytes = int(free * hbm_fraction)
-            self._runner = runner
-        else:
-            self._runner = None
-            if slot_states is None:
-                slot_states = [None] * slots
+        self._runner = None
+        native_ctx = run_tile is None
+        if native_ctx:
+            self._runner = runner if runner is not None else GpuTileRunner(device)
+            run_tile = self._runner.run
+        elif slot_states is None:
+            slot_states = [None] * slots
         self._run = run_tile
-        self.budget = budget_bytes if budget_bytes is not None else 1 << 62
-        self._slots = [_Slot(q, s) for q, s in enumerate(slot_states)]
-        self._pending = collections.deque()
-        self._reserved = 0
-        self._cv = threading.Condition()
+        self._lock = threading.Lock()
+        self._reqs = {}
+        self._next_id = 0
         self._closed = False
-        self.max_in_flight = 0
-        self._in_flight = 0
-        self._sched = threading.Thread(target=self._schedule, name="tile-scheduler", daemon=True)
-        self._sched.start()
+        self._unresolved = 0
+        self._cb = L.TILE_FN(self._call)
+        cfg = L.ServeConfig(device if native_ctx else -1, slots, queue_capacity,
+                            0 if (native_ctx and budget_bytes is None) else int(budget_bytes or (1 << 62)),
+                            float(hbm_fraction), self._cb, None)
+        h = C.c_void_p()
+        L.check(self._lib.pp_serve_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._t0 = time.perf_counter()
+        if native_ctx:
+            slot_states = []
+            for q in range(slots):
+                c = C.c_void_p()
+                L.check(self._lib.pp_serve_context(self._h, q, C.byref(c)))
+                slot_states.append(self._runner.make_slot(handle=c.value))
+        self._states = slot_states
+        self.budget = self.stats()["budget"]
+        self._done_thread = threading.Thread(target=self._collect, name="tile-completions", daemon=True)
+        self._done_thread.start()
 
     # ---------------------------------------------------------------- API
     def submit(self, req: TileRequest) -> Future:
+        import ctypes as C
         fut: Future = Future()
         req.arrival = time.perf_counter()
-        with self._cv:
+        with self._lock:
             if self._closed:
                 raise RuntimeError("server closed")
-            self._pending.append((req, fut))
-            self._cv.notify_all()
+            rid = self._next_id
+            self._next_id += 1
+            self._reqs[rid] = (req, fut)
+            self._unresolved += 1
+        need = int(self._footprint(req.nmesh))
+        t = C.c_int64()
+        rc = self._lib.pp_serve_submit(self._h, C.c_void_p(rid), need, C.byref(t))
+        if rc != 0:
+            msg = self._lib.pp_last_error().decode()
+            with self._lock:
+                self._reqs.pop(rid)
+                self._unresolved -= 1
+            if rc == -6:   # PP_ERR_CAPACITY: could never be admitted
+                now = time.perf_counter()
+                fut.set_result(TileResult(req.tile_id, req.nmesh, arrival=req.arrival, start=now, done=now,
+                                          error=f"AdmissionError: {msg}"))
+                return fut
+            self._L.check(rc)
+        fut.ticket = t.value
         return fut
 
+    def inject_fault(self, fut: Future):
+        """Make a queued tile fail with PP_ERR_INJECTED instead of running
+        (failure tests)."""
+        self._L.check(self._lib.pp_serve_inject_fault(self._h, fut.ticket))
+
+    def stats(self) -> dict:
+        import ctypes as C
+        s = (C.c_int64 * 7)()
+        self._L.check(self._lib.pp_serve_stats(self._h, s))
+        return dict(zip(("waiting", "in_flight", "max_in_flight", "finished", "failed", "reserved", "budget"),
+                        list(s)))
+
+    @property
+    def max_in_flight(self) -> int:
+        return self._max_in_flight if self._h is None else self.stats()["max_in_flight"]
+
     def close(self):
-        with self._cv:
+        with self._lock:
+            if self._closed:
+                return
             self._closed = True
-            self._cv.notify_all()
-        self._sched.join()
-        # wait for running tiles
-        with self._cv:
-            while self._in_flight:
-                self._cv.wait()
+        self._done_thread.join()
+        self._max_in_flight = self.stats()["max_in_flight"]
+        self._lib.pp_serve_destroy(self._h)
+        self._h = None
         if self._runner is not None:
-            for s in self._slots:
-                self._runner.release_slot(s.state)
+            for s in self._states:
+                self._runner.release_slot(s)
 
     def __enter__(self):
         return self
@@ -177,48 +224,45 @@ class TileServer:
     def __exit__(self, *exc):
         self.close()
 
-    # ---------------------------------------------------------- scheduler
-    def _schedule(self):
-        with self._cv:
-            while True:
-                while self._pending:
-                    req, fut = self._pending[0]
-                    need = self._footprint(req.nmesh)
-                    if need > self.budget:
-                        self._pending.popleft()
-                        fut.set_result(TileResult(req.tile_id, req.nmesh, arrival=req.arrival,
-                                                  start=time.perf_counter(), done=time.perf_counter(),
-                                                  error=f"AdmissionError: tile needs {need} B of HBM, "
-                                                        f"budget {self.budget} B"))
-                        continue
-                    slot = next((s for s in self._slots if not s.busy), None)
-                    if slot is None or self._reserved + need > self.budget:
-                        break  # FIFO: the head waits for a slot / memory
-                    self._pending.popleft()
-                    slot.busy = True
-                    self._reserved += need
-                    self._in_flight += 1
-                    self.max_in_flight = max(self.max_in_flight, self._in_flight)
-                    threading.Thread(target=self._work, args=(slot, req, fut, need), daemon=True,
-                                     name=f"tile-slot{slot.index}").start()
-                if self._closed and not self._pending:
-                    return
-                self._cv.wait()
-
-    def _work(self, slot: _Slot, req: TileRequest, fut: Future, need: int):
-        start = time.perf_counter()
+    # ------------------------------------------------------------ callbacks
+    def _call(self, user, ctx, slot, ticket, request):
+        """A slot thread of the native server runs one tile."""
+        rid = int(request or 0)
+        req, _ = self._reqs[rid]
         try:
-            res = self._run(slot.state, req)
+            res = self._run(self._states[slot], req)
         except Exception as e:  # the tile fails, the server keeps serving
             res = TileResult(req.tile_id, req.nmesh, error=f"{type(e).__name__}: {e}")
-        res.arrival, res.start, res.slot = req.arrival, start, slot.index
-        res.done = time.perf_counter()
-        with self._cv:
-            slot.busy = False
-            self._reserved -= need
-            self._in_flight -= 1
-            self._cv.notify_all()
-        fut.set_result(res)
+        self._reqs[rid] = (req, self._reqs[rid][1], res)
+        return 0 if res.error is None else -1
+
+    def _collect(self):
+        """Resolve futures in completion order."""
+        import ctypes as C
+        t, st, rq = C.c_int64(), C.c_int(), C.c_void_p()
+        times = (C.c_double * 4)()
+        buf = C.create_string_buffer(1024)
+        while True:
+            with self._lock:
+                if self._closed and self._unresolved == 0:
+                    return
+            rc = self._lib.pp_serve_next(self._h, 100, C.byref(t), C.byref(rq), C.byref(st), times)
+            if rc == -8:   # PP_ERR_TIMEOUT
+                continue
+            self._L.check(rc)
+            # the request id travels as the opaque request pointer
+            entry = self._reqs.pop(int(rq.value or 0))
+            req, fut = entry[0], entry[1]
+            res = entry[2] if len(entry) > 2 else None
+            if res is None:
+                self._lib.pp_serve_message(self._h, t.value, buf, len(buf))
+                name = self._L.ERRORS.get(st.value, str(st.value))
+                res = TileResult(req.tile_id, req.nmesh, error=f"{name}: {buf.value.decode()}")
+            res.arrival = req.arrival
+            res.start, res.done, res.slot = self._t0 + times[1], self._t0 + times[2], int(times[3])
+            with self._lock:
+                self._unresolved -= 1
+            fut.set_result(res)
 
 
 def replay(server: TileServer, stream, time_scale: float = 1.0):
@@ -263,10 +307,13 @@ class GpuTileRunner:
         self.table = table if table is not None else collapse.sinusoidal_table()
         self.tunables = tunables
 
-    def make_slot(self):
+    def make_slot(self, handle=None):
+        """A slot's state: its Context (a new one, or a non-owning wrapper of
+        the native server's slot context `handle`) with the tables loaded,
+        and a torch stream on the context's stream."""
         import torch
         from .api import Context
-        ctx = Context(self.device)
+        ctx = Context(self.device) if handle is None else Context.from_handle(handle, self.device)
         ctx.upload_collapse_table(self.table.table, *self.table.bounds)
         ctx.build_shell_tables(self.template.nhunt)
         if self.tunables:

@@ -13,7 +13,7 @@ fields of the configured shape:
   no-wiggle transfer function, n_s tilt, sigma_8 normalised with the real-space
   top-hat (MassFunction.jl:29-42 convention), then divided by (2 pi)^3 so that
   <delta^2> = int P d^3k, the field-generator convention of
-  RandomField.jl:388 (the reference reads P(k) from a file in that convention).
+  RandomField.jl:136-175 (the reference reads P(k) from a file in that convention).
 * ``generate_grf`` (RandomField.jl:123-178), ``displacements_1lpt`` /
   ``displacements_2lpt`` (LPT.jl:21-191), ``compute_laplacian``
   (Pipeline.jl:35-51).

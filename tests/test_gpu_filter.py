@@ -158,7 +158,7 @@ def test_filter_bank_vs_oracle(ctx):
 
 def test_threefry_noise_kernel_matches_host(ctx):
     """pp_fill_noise_threefry against the host restatement of
-    fill_noise_threefry! (RandomField.jl:269-306), odd cell count included."""
+    fill_noise_threefry! (RandomField.jl:46-81), odd cell count included."""
     import torch
     from peakpatch_hip import synth
     n = 33

@@ -69,7 +69,7 @@ def test_power_spectrum_normalisation():
     """EH98 no-wiggle P(k) normalised to sigma8 (MassFunction.jl:29-42)."""
     pk = synth.EH98NoWiggle()
     assert pk.sigma_R(8.0) == pytest.approx(0.808, rel=1e-4)
-    # field-generator convention: P / (2 pi)^3 (RandomField.jl:388, quirk Q18)
+    # field-generator convention: P / (2 pi)^3 (RandomField.jl:136-175, quirk Q18)
     k = np.array([0.05, 0.1, 0.5])
     np.testing.assert_allclose(pk(k), pk.p_std(k) / (2 * math.pi) ** 3, rtol=1e-12)
 

@@ -84,3 +84,90 @@ def test_submit_after_close_raises():
     srv.close()
     with pytest.raises(RuntimeError):
         srv.submit(TileRequest(0, 128))
+
+
+# ---- the native serving entry points through the C ABI (no GPU: device -1) ----
+import ctypes as C  # noqa: E402
+
+from peakpatch_hip import _lib as L  # noqa: E402
+
+
+class _Native:
+    """pp_serve_* with a ctypes tile function that sleeps `service_s` and
+    records which requests ran."""
+
+    def __init__(self, slots=2, queue=64, budget=1 << 40, service_s=0.05):
+        self.lib = L.load()
+        self.ran = []
+        self.lock = threading.Lock()
+
+        def fn(user, ctx, slot, ticket, request):
+            assert ctx is None
+            time.sleep(service_s)
+            with self.lock:
+                self.ran.append(int(request or 0))
+            return 0
+        self.fn = L.TILE_FN(fn)
+        cfg = L.ServeConfig(-1, slots, queue, budget, 0.0, self.fn, None)
+        self.h = C.c_void_p()
+        L.check(self.lib.pp_serve_create(C.byref(cfg), C.byref(self.h)))
+
+    def submit(self, rid, need=1):
+        t = C.c_int64()
+        rc = self.lib.pp_serve_submit(self.h, C.c_void_p(rid), need, C.byref(t))
+        return rc, t.value
+
+    def poll(self, t, timeout_ms=-1):
+        st = C.c_int()
+        times = (C.c_double * 4)()
+        rc = self.lib.pp_serve_poll(self.h, t, timeout_ms, C.byref(st), times)
+        return rc, st.value, list(times)
+
+    def close(self):
+        L.check(self.lib.pp_serve_destroy(self.h))
+
+
+def test_native_injected_fault_leaves_other_tiles_intact():
+    s = _Native(slots=2)
+    try:
+        tickets = [s.submit(100 + q)[1] for q in range(6)]
+        L.check(s.lib.pp_serve_inject_fault(s.h, tickets[4]))
+        out = [s.poll(t) for t in tickets]
+        assert all(rc == 0 for rc, _, _ in out)
+        assert [st for _, st, _ in out] == [0, 0, 0, 0, -9, 0]
+        assert sorted(s.ran) == [100, 101, 102, 103, 105]
+        buf = C.create_string_buffer(256)
+        L.check(s.lib.pp_serve_message(s.h, tickets[4], buf, 256))
+        assert b"fault injected" in buf.value
+        stats = (C.c_int64 * 7)()
+        L.check(s.lib.pp_serve_stats(s.h, stats))
+        assert stats[3] == 6 and stats[4] == 1 and stats[2] == 2
+    finally:
+        s.close()
+
+
+def test_native_busy_timeout_and_capacity():
+    s = _Native(slots=1, queue=2, budget=1000, service_s=0.3)
+    try:
+        rc0, t0 = s.submit(1)          # starts at once
+        time.sleep(0.05)
+        assert s.submit(2)[0] == 0 and s.submit(3)[0] == 0
+        assert s.submit(4)[0] == -7    # PP_ERR_BUSY: two already wait
+        assert s.submit(5, need=1001)[0] == -6   # PP_ERR_CAPACITY: above the budget
+        rc, st, _ = s.poll(t0, timeout_ms=10)
+        assert rc == -8                # PP_ERR_TIMEOUT: still running
+        rc, st, times = s.poll(t0)
+        assert rc == 0 and st == 0 and times[2] >= times[1] >= times[0]
+    finally:
+        s.close()
+    assert s.ran == [1, 2, 3]
+
+
+def test_tileserver_inject_fault():
+    run, _ = fake_runner(service_s=0.05)
+    with TileServer(slots=1, run_tile=run) as srv:
+        futs = [srv.submit(TileRequest(q, 128)) for q in range(4)]
+        srv.inject_fault(futs[3])
+        res = [f.result(timeout=30) for f in futs]
+    assert [r.error is None for r in res] == [True, True, True, False]
+    assert res[3].error.startswith("PP_ERR_INJECTED")
