@@ -155,6 +155,23 @@ int pp_find_peaks_field(pp_ctx *ctx, const float *delta_s, int8_t *mask, int on_
                         int n1, int n2, int n3, int nbuff, float fcrit, int64_t max_peaks,
                         pp_peaks *out);
 
+/* pp_find_peaks_field for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same stencil, order, max_peaks cut and mask
+ * update as pp_find_peaks_field; every pointer is device memory of the
+ * context's device and every array of `out` (i, j, k, filt, delta_s; lapd_s
+ * is ignored) is required.  The call allocates nothing: its counters, peak
+ * bit mask, offsets and scan storage live in a static device array of
+ * PP_NOALLOC_SCRATCH_BYTES per device (held until the process ends), which
+ * limits the inner cube to about 8 M 64-cell row words (ceil(in1 / 64) x in2
+ * x in3; any nbuff at n = 768); larger grids are refused with PP_ERR_ARG
+ * before any work is queued.  Calls that overlap on one GPU share that array
+ * and race: serialise them (one stream, or one call at a time).  Blocks until
+ * the counts are back (out->count, out->needed). */
+#define PP_NOALLOC_SCRATCH_BYTES (96u << 20)
+int pp_find_peaks_field_noalloc(pp_ctx *ctx, const float *delta_s, int8_t *mask, int n1, int n2, int n3,
+                                int nbuff, float fcrit, int64_t max_peaks, pp_peaks *out);
+
 /* peak_find_tile_gpu (ext/CUDAExt.jl:3068-3197; src/PeakPatch.jl:341-366) and
  * the CPU filter loop (src/Pipeline.jl:160-195; MultiResolution.jl:885-907):
  * rfft(delta) once, then per filter in the given order (the driver passes

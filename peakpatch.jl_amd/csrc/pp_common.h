@@ -155,5 +155,6 @@ struct pp_ctx {
 namespace pp {
 int fft_prepare(pp_ctx *ctx, int n);
 void persist_release(pp_ctx *ctx);    // pp_persist.hip: drops the context's gather tables
+int noalloc_scratch_init(int device);  // pp_filter.hip: device address of the static scratch
 int ensure_device(pp_ctx *ctx);
 }  // namespace pp

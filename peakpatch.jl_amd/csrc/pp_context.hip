@@ -129,6 +129,11 @@ int pp_ctx_create(int device, pp_ctx **out)
     }
     c->num_cu = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
+    int rc = noalloc_scratch_init(device);  // once per device, with the device current
+    if (rc != PP_OK) {
+        pp_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return PP_OK;
 }

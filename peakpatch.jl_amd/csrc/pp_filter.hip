@@ -22,6 +22,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <mutex>
 
 #include "pp_common.h"
 
@@ -371,6 +372,44 @@ static int run_stencil(pp_ctx *ctx, const float *d_field, int8_t *d_mask, const 
                             stencil_carve(ctx->d_scan.ptr, g, tmp));
 }
 
+// ------------------------------------------- peak finding without allocation
+// pp_find_peaks_field_noalloc keeps its counters and the stencil's bit mask,
+// offsets and scan storage in `scratch`, a file-scope array every device the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel: it passes the device address hipGetSymbolAddress
+// gives for the current device, taken once per device when a context is
+// created there (noalloc_scratch_init).
+static __device__ __align__(256) unsigned char scratch[PP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+int noalloc_scratch_init(int device)
+{
+    PP_ARG(device >= 0 && device < kMaxDevices, "device %d beyond the %d the library tracks", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PP_OK;
+    void *p = nullptr;
+    PP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PP_ARG(p != nullptr, "no device address for the scratch array");
+    g_scratch_ptr[device] = p;
+    return PP_OK;
+}
+
+static void *noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+// clears the counters with a kernel rather than a memset node, so that a
+// caller may capture the call in a HIP graph
+__global__ void k_zero_counters(long long *c)
+{
+    if (threadIdx.x < 4) c[threadIdx.x] = 0;
+}
+
 static int alloc_peak_out(pp_ctx *ctx, long long cap, PeakOut &o, float **lapd, long long **counters)
 {
     size_t per = cap > 0 ? (size_t)cap : 1;
@@ -598,6 +637,54 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
     ctx->timing.peakfind_ms = ms;
     return rc;
+}
+
+int pp_find_peaks_field_noalloc(pp_ctx *ctx, const float *delta_s, int8_t *mask, int n1, int n2, int n3, int nbuff,
+                                float fcrit, int64_t max_peaks, pp_peaks *out)
+{
+    PP_ARG(ctx && delta_s && mask && out, "NULL argument");
+    PP_ARG(out->i && out->j && out->k && out->filt && out->delta_s, "every output array is required");
+    PP_ARG(n1 >= 3 && n2 >= 3 && n3 >= 3, "grid %dx%dx%d too small", n1, n2, n3);
+    PP_ARG(nbuff >= 1, "nbuff must be >= 1 (the 27-point stencil reads one cell past the inner cube)");
+    PP_ARG(out->capacity >= 0, "negative capacity");
+    PP_CHECK(ensure_device(ctx));
+    if (max_peaks <= 0) max_peaks = 2000000;
+    const StencilGeom g = stencil_geom(n1, n2, n3, nbuff);
+    size_t need = 0, tmp = 0;
+    PP_CHECK(stencil_scan_bytes(g, &tmp));
+    PP_CHECK(stencil_scratch_bytes(g, &need));
+    need += 256;
+    PP_ARG(need <= (size_t)PP_NOALLOC_SCRATCH_BYTES,
+           "inner cube %dx%dx%d needs %zu B of scratch, more than the %zu B this entry point has; use "
+           "pp_find_peaks_field", g.in1, g.in2, g.in3, need, (size_t)PP_NOALLOC_SCRATCH_BYTES);
+    char *base = (char *)noalloc_scratch(ctx->device);
+    if (!base) {
+        set_error("no scratch address for device %d (context creation records it)", ctx->device);
+        return PP_ERR_STATE;
+    }
+    long long *d_counters = (long long *)base;
+    PeakOut o;
+    o.i = out->i; o.j = out->j; o.k = out->k; o.filt = out->filt; o.delta_s = out->delta_s;
+    o.capacity = out->capacity;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, d_counters);
+    PP_HIP(hipGetLastError());
+    PP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    PP_CHECK(run_stencil_with(ctx, delta_s, mask, g, fcrit, max_peaks, 0, o, d_counters,
+                              stencil_carve(base + 256, g, tmp)));
+    PP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    long long cnt[4];
+    PP_HIP(hipMemcpyAsync(cnt, d_counters, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->timing.peakfind_ms = ms;
+    out->needed = cnt[1];
+    out->count = cnt[1] < out->capacity ? cnt[1] : out->capacity;
+    if (cnt[1] > out->capacity) {
+        set_error("peak output capacity %lld too small: %lld peaks found", (long long)out->capacity, cnt[1]);
+        return PP_ERR_CAPACITY;
+    }
+    return PP_OK;
 }
 
 }  // extern "C"

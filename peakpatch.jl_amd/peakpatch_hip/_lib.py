@@ -93,6 +93,7 @@ SIGNATURES = {
     "pp_smooth_field": (_i, [_vp, _vp, _vp, _i, _i, _d, _d, _i]),
     "pp_rfft": (_i, [_vp, _vp, _vp, _i, _i]),
     "pp_find_peaks_field": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _i64, C.POINTER(Peaks)]),
+    "pp_find_peaks_field_noalloc": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _i64, C.POINTER(Peaks)]),
     "pp_find_peaks": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _i, _i, _i, _i, _i64, _vp, C.POINTER(Peaks)]),
     "pp_analyse_peaks": (_i, [_vp, C.POINTER(Fields), C.POINTER(PeakBatch), _d, _d, _i, _vp, _i, _i,
                               C.POINTER(Results)]),
