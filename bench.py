@@ -147,28 +147,43 @@ def read_traffic(path, head, kernel):
 
 
 def serve_stream(args, ct, tuned):
-    """The c5 serving workload: a Poisson stream of light-cone tiles with
-    nmesh drawn from {128, 256, 384, 512, 768} at the c2 resolution, each
-    synthesised, peak-found, shell-analysed and packed on the GPU; several
-    tiles in flight on separate slots (streams).  Latency includes queueing."""
+    """The c5 serving workload: light-cone tiles with nmesh drawn from
+    {128, 256, 384, 512, 768} at the c2 resolution, each synthesised,
+    peak-found, bookkept, shell-analysed and packed on the GPU, on the native
+    server's slot contexts with admission by device memory.  Two runs of
+    args.serve_tiles tiles: all submitted at once (the drain rate is the
+    saturation capacity), then Poisson arrivals at args.serve_load times that
+    capacity (or at args.serve_rate); latency = completion - arrival,
+    queueing included."""
     from peakpatch_hip import serve
     runner = serve.GpuTileRunner(table=ct, tunables=tuned)
-    stream = serve.poisson_stream(args.serve_tiles, args.serve_rate, seed=5)
-    # warm every slot once (plans, pools) with a small tile, untimed
-    slots = [runner.make_slot() for _ in range(args.serve_slots)]
-    for st in slots:
-        runner.run(st, serve.TileRequest(-1, 128, (400.0, 400.0, 400.0), seed=1))
-    with serve.TileServer(slots=len(slots), run_tile=runner.run, slot_states=slots) as srv:
+    n = args.serve_tiles
+    with serve.TileServer(slots=args.serve_slots, runner=runner, hbm_fraction=0.8) as srv:
+        # warm the slots on every nmesh (transform plans, buffer pools), untimed
+        warm = [serve.TileRequest(-1, m, (400.0, 400.0, 400.0), seed=1)
+                for m in serve.SERVE_NMESH for _ in range(args.serve_slots)]
+        bad = [r.error for r in serve.replay(srv, [(0.0, r) for r in warm]) if r.error]
+        if bad:
+            raise RuntimeError(f"serving warm-up failed: {bad[0]}")
+        burst = serve.poisson_stream(n, 0.0, seed=5)
+        cap = serve.summarize(serve.replay(srv, burst))
+        rate = args.serve_rate if args.serve_rate > 0 else args.serve_load * cap["tiles_per_s"]
+        stream = serve.poisson_stream(n, rate, seed=6)
         res = serve.replay(srv, stream)
-        inflight = srv.max_in_flight
-    for st in slots:
-        runner.release_slot(st)
+        st = srv.stats()
     s = serve.summarize(res)
-    s.update({"workload": f"c5: {args.serve_tiles} tiles, Poisson {args.serve_rate:g}/s, nmesh from "
-                          f"{serve.SERVE_NMESH}, alatt 250/256 Mpc/h, 20 filters 1-15 Mpc/h, 2LPT, light-cone",
-              "slots": len(slots), "max_in_flight": inflight,
-              "nmesh_drawn": [r.nmesh for _, r in stream]})
-    if s["failed"]:
+    mesh = [r.nmesh for _, r in stream]
+    s.update({"workload": f"c5: {n} tiles, Poisson {rate:.3g}/s, nmesh from {serve.SERVE_NMESH}, alatt 250/256 "
+                          f"Mpc/h, 20 filters 1-15 Mpc/h, 2LPT, light-cone",
+              "arrival_rate_per_s": rate,
+              "load": rate / cap["tiles_per_s"] if cap["tiles_per_s"] > 0 else None,
+              "capacity_tiles_per_s": cap["tiles_per_s"],
+              "capacity_run": {k: cap[k] for k in ("tiles", "completed", "failed", "wall_s", "p50_ms", "p99_ms")},
+              "slots": args.serve_slots, "max_in_flight": st["max_in_flight"],
+              "admission": {"budget_bytes": st["budget"], "rule": "footprint within 0.8 of the free HBM at start, "
+                            "and within the live free HBM while others run"},
+              "nmesh_counts": {str(m): mesh.count(m) for m in serve.SERVE_NMESH}})
+    if s["failed"] or cap["failed"]:
         s["errors"] = sorted({r.error for r in res if r.error})
     return s
 
@@ -189,9 +204,12 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="before the timed steps, run one elementwise kernel over a GIB-sized float32 buffer "
                          "(known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--serve-tiles", type=int, default=12,
-                    help="tiles in the serving stream (Poisson arrivals, nmesh in {128..768}); 0 skips it")
-    ap.add_argument("--serve-rate", type=float, default=0.4, help="tile arrival rate, tiles/s")
+    ap.add_argument("--serve-tiles", type=int, default=200,
+                    help="tiles in each serving run (nmesh in {128..768}); 0 skips serving")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="Poisson arrival rate of the latency run, as a fraction of the measured capacity")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrival rate in tiles/s (overrides --serve-load when > 0)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream (a test of it)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_gather_c3_pmc.txt"),
@@ -215,8 +233,7 @@ def main():
         return
     ctx.build_shell_tables(cfg.nhunt)
     t0 = time.perf_counter()
-    tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1,
-                                 device="cuda", ctx=ctx)
+    tile = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
     ctx.synchronize()
     torch.cuda.synchronize()
     log(f"[bench] {cfg.describe()}; fields synthesised in {time.perf_counter() - t0:.1f} s")

@@ -183,6 +183,48 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
                   const double *Rf, const float *fcrit, int nfilt, int wsmooth, int want_lapd,
                   int nbuff, int64_t max_peaks_per_filter, int8_t *mask_inout, pp_peaks *out);
 
+/* ------------------------------------------------ per-peak bookkeeping */
+/* What the drivers know about a tile when they turn peaks into a batch
+ * (src/Pipeline.jl:217-231, 257-271; src/MultiResolution.jl:959-988). */
+typedef struct {
+    int n;                  /* tile mesh                                      */
+    double alatt;           /* cell size, Mpc/h                               */
+    double xbx, ybx, zbx;   /* tile centre, Mpc/h                             */
+    int ievol;              /* 0: every peak at z_out; 1: light cone from obs */
+    double z_out, obs[3], z_max;
+    int ilpt;               /* >= 2: the 2LPT velocity factor is formed       */
+    double Om, OL;
+    int round_f32;          /* 1: ZZon and fcrit rounded through Float32 as
+                               run_multitile_split does (MultiResolution.jl:
+                               963-967); 0: Float64 as run_tile              */
+    pp_growth growth;       /* host tables                                    */
+    int nchi;               /* chi -> z table (build_chi_to_z, Cosmology.jl:  */
+    const double *chi_z, *chi_chi; /* 333-337), host, ievol = 1 only         */
+    double chi_zmax;        /* the table's z_max                             */
+} pp_tile_geom;
+
+/* Device arrays, one row per kept peak (capacity: the peak count). */
+typedef struct {
+    int32_t *i, *j, *k;
+    double *Rf;
+    int64_t *ir2min;
+    double *ZZon, *fcrit, *x, *y, *z, *D, *v2fac;
+    float *FcRf, *d2Rf;
+} pp_batch_out;
+
+/* The drivers' per-peak loop on the device: for the peaks of pp_find_peaks
+ * (device arrays; peaks->count rows) the filter radius Rf_filters[filt]
+ * (host array of nfilt), position, redshift (z_out or peak_redshift through
+ * the chi -> z table, Cosmology.jl:344-372), ZZon = 1 + z, fcrit =
+ * fsc_of_z(z) (RadialShell.jl:191-196), ir2min, growth factor D_a
+ * (Dlinear_ab, Cosmology.jl:273-293) and 2LPT velocity factor
+ * 3/7 Om_a^(-1/143) D_a^2; light-cone peaks with z > z_max are dropped, the
+ * others kept in order (*nkept rows written).  The output feeds
+ * pp_analyse_peaks (i, j, k, Rf, ir2min, ZZon, fcrit) and pp_pack_halos
+ * (x, y, z, Rf, ZZon, D, v2fac, FcRf, d2Rf) without leaving the device. */
+int pp_bookkeep_peaks(pp_ctx *ctx, const pp_peaks *peaks, const double *Rf_filters, int nfilt,
+                      const pp_tile_geom *geom, pp_batch_out *out, int64_t *nkept);
+
 /* --------------------------------------------------------- shell analysis */
 typedef struct {
     int n1, n2, n3;
@@ -276,6 +318,76 @@ int pp_last_timing(pp_ctx *ctx, double *shell_gather_ms, double *shell_total_ms,
  * Float64.  The tile synthesis uses it; it is not on the analysis path. */
 int pp_fill_noise_threefry(pp_ctx *ctx, float *out, int64_t ncells, uint64_t seed);
 
+/* P(k) as a table (the two columns of the reference's P(k) file, load_pk,
+ * src/InitialConditions/PowerSpectrum.jl:11-23): log P is interpolated
+ * linearly in log k and extrapolated linearly beyond the ends.  Host arrays,
+ * k increasing, both positive. */
+typedef struct {
+    int npk;
+    const double *k, *P;
+} pp_pk_table;
+
+/* The fields of one tile, device arrays of n^3 Float32 (x fastest).  eta2x..z
+ * are all NULL (no 2LPT) or all set; lapd may be NULL. */
+typedef struct {
+    float *delta, *etax, *etay, *etaz, *eta2x, *eta2y, *eta2z, *lapd;
+} pp_tile_fields;
+
+/* The inputs run_tile builds for a periodic tile (src/Pipeline.jl:110-206):
+ * generate_grf (src/InitialConditions/RandomField.jl:123-175: Threefry noise
+ * of fill_noise_threefry!, rfft, every mode times sqrt(P(k) dk^3 n^3) with
+ * k = 0 zeroed, normalised irfft), then from delta_k = rfft(delta) the 1LPT
+ * and 2LPT displacements (src/InitialConditions/LPT.jl:21-191, Nyquist planes
+ * zeroed) and the k^2 field of _compute_laplacian (src/Pipeline.jl:35-51).
+ * Runs on the context stream and returns when the fields are written. */
+int pp_make_tile_fields(pp_ctx *ctx, int n, double boxsize, uint64_t seed, const pp_pk_table *pk,
+                        pp_tile_fields *fields);
+
+/* ------------------------------------------- multi-resolution tile fields */
+/* Geometry of run_multitile_split (src/MultiResolution.jl:546-553, 596-613):
+ * tiles of nmesh^3 cells with a buffer of nbuff, nsub = nmesh - 2 nbuff core
+ * cells, ntile^3 tiles of a global N^3 grid, N = nsub ntile + 2 nbuff; a
+ * coarse M^3 grid (coarse_factor x ntile, or coarse_grid) that divides N. */
+typedef struct {
+    int nmesh, nbuff, ntile, M;
+    double alatt;           /* cell size, Mpc/h                               */
+    uint64_t seed;          /* Threefry key of the global white noise         */
+} pp_split_geom;
+
+/* The coarse fields, device-resident (any context on the same device may
+ * use them once pp_coarse_create has returned). */
+typedef struct pp_coarse pp_coarse;
+
+/* Phase 1a of run_multitile_split (src/MultiResolution.jl:617-631): the
+ * coarse noise of _downsample_noise (:58-83: block sums of the global
+ * Threefry normals in Float64 over sqrt(block^3), Float32) and its periodic
+ * convolutions (_periodic_convolve!, :288-307) delta_coarse and
+ * psi_coarse[1..3] (sqrt(P(k) dk^3 M^3), times i k_d / k^2 for psi, k = 0
+ * zeroed, normalised irfft).  Runs on ctx's stream and returns when done. */
+int pp_coarse_create(pp_ctx *ctx, const pp_split_geom *geom, const pp_pk_table *pk, pp_coarse **out);
+
+/* Host copies of the coarse fields (M^3 Float32 each, x fastest); any
+ * pointer may be NULL. */
+int pp_coarse_download(const pp_coarse *coarse, float *noise, float *delta, float *psix, float *psiy, float *psiz);
+
+int pp_coarse_destroy(pp_coarse *coarse);
+
+/* One tile's fields on process_tile!'s path (src/MultiResolution.jl:
+ * 719-850; tile indices it, jt, kt 1-based): the Hoffman-Ribak residual
+ * (_generate_extended_residual, :197-236: fine normal minus its coarse
+ * block mean), its isolated convolution (_isolated_convolve, :246-283: zero
+ * padded to (2 nmesh)^3, sqrt(P(k) dk^3 (2 nmesh)^3) [times i k_d / k^2],
+ * k = 0 zeroed, the kx = 0 and Nyquist planes made Hermitian as the
+ * reference's GPU path does, ext/CUDAExt.jl:2561-2611), plus the coarse
+ * field's Catmull-Rom interpolation (_interpolate_to_tile / _tricubic,
+ * :95-122, 156-187) for delta and psi; then from the combined delta the
+ * tile-periodic 2LPT displacements (src2 = delta^2/2 - sum phi_dd^2/2 -
+ * sum_{a<b} phi_ab^2, :806-831) and k^2 field (:843-847) with
+ * _apply_kernel_inplace!'s zeroing of k = 0 and the Nyquist planes
+ * (:1239-1255).  fields: device arrays of nmesh^3 (eta2x..z all NULL to
+ * skip 2LPT, lapd NULL to skip it).  coarse was made on ctx's device. */
+int pp_split_tile_fields(pp_ctx *ctx, const pp_coarse *coarse, int it, int jt, int kt, pp_tile_fields *fields);
+
 /* ------------------------------------------------------- halo records */
 /* Per-peak inputs of the record packing besides the pp_results: the host
  * bookkeeping of the drivers (Pipeline.jl:217-231, 257-271).  Device arrays. */
@@ -303,6 +415,19 @@ typedef struct {
 int pp_pack_halos(pp_ctx *ctx, const pp_results *res, const pp_halo_inputs *in, double alatt, double z_max,
                   int extended, float *records, int64_t capacity, int records_on_device, int64_t *nhalo,
                   float *RTHLmax);
+
+/* merge_catalog (src/Merger/Merger.jl:21-79; src/Merger/Exclusion.jl:14-222)
+ * over n halos given by their catalogue columns x, y, z and RTHL (Float32,
+ * device arrays when on_device, else host): Lagrangian exclusion in
+ * descending RTHL order (stable: a surviving halo removes every not larger
+ * halo whose centre lies inside its RTHL), then volume reduction of the
+ * survivors (sphere_overlap caps of every overlapping pair,
+ * r_new = ((4/3 pi r^3 - dV) / (4/3 pi))^(1/3), halos left without volume
+ * removed).  keep[i] = 1 for the halos of the merged catalogue, whose RTHL
+ * becomes new_rthl[i] (0 for removed halos); *nkept counts them.  keep and
+ * new_rthl are device arrays when on_device, else host. */
+int pp_merge_halos(pp_ctx *ctx, int64_t n, const float *x, const float *y, const float *z, const float *rthl,
+                   int on_device, int8_t *keep, float *new_rthl, int64_t *nkept);
 
 /* ------------------------------------------------------------- serving */
 /* A tile server on one GPU: the per-device task loop of run_multitile_split

@@ -2,8 +2,8 @@
 
     python peakpatch.jl_amd/build.py          # build lib/libpeakpatch_hip.so
 
-PP_BUILD_OUT (output path) and PP_EXTRA_CFLAGS (extra hipcc flags, e.g. the
-tuning macros of pp_post.h) build variants beside the default library.
+PP_BUILD_OUT (output path) and PP_EXTRA_CFLAGS (extra hipcc flags, e.g. -g)
+build variants beside the default library.
 
 Every HIP source under csrc/ is compiled to an object (in parallel) and the
 objects are linked with rocFFT into lib/libpeakpatch_hip.so.  All device code

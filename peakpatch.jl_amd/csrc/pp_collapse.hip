@@ -19,15 +19,11 @@
 #include <vector>
 
 #include "pp_common.h"
+#include "pp_growth.h"
 
 namespace pp {
 
 constexpr int kGrowthMax = 2048;
-
-struct GrowthDev {
-    int ntab;
-    double amin, amax, D_inf, D_0, HD_inf, HD_0;
-};
 
 struct EllState {
     double Rvac, Rcurv;
@@ -35,54 +31,6 @@ struct EllState {
     double a_eq[3], a_3eq2, a_3eq1, Ha;
     double lam[3], F3;
 };
-
-__device__ __forceinline__ int hunt_lds(const double *xx, int n, double x)
-{
-    // the reference's HUNT bisection (ascending table), 1-based result
-    int jlo = 0, jhi = n + 1;
-    while (jhi - jlo > 1) {
-        const int jm = (jhi + jlo) >> 1;
-        if (x > xx[jm - 1]) jlo = jm;
-        else jhi = jm;
-    }
-    return jlo;
-}
-
-// Dlinear_ab (src/Cosmology/Cosmology.jl:273-293): (D, dlnD/dlna), the
-// 4-point Lagrange rule inside the table, 2-point next to its ends
-__device__ void dlinear(const double *at, const double *Dt, const double *Ht, const GrowthDev &g, double a,
-                        double &D, double &HD)
-{
-    double Da;
-    if (a > g.amin && a < g.amax) {
-        const int kk = hunt_lds(at, g.ntab, a);
-        int nl, kl;
-        if (kk < 2 || kk >= g.ntab) { nl = 2; kl = kk - 1 > 1 ? kk - 1 : 1; }
-        else { nl = 4; kl = kk - 2; }
-        if (kl < 1) { nl = 2; kl = 1; }
-        const double *xa = at + kl - 1, *v1 = Dt + kl - 1, *v2 = Ht + kl - 1;
-        if (nl == 4) {
-            const double a1 = (a - xa[1]) / (xa[0] - xa[1]) * (a - xa[2]) / (xa[0] - xa[2]) * (a - xa[3]) / (xa[0] - xa[3]);
-            const double a2 = (a - xa[0]) / (xa[1] - xa[0]) * (a - xa[2]) / (xa[1] - xa[2]) * (a - xa[3]) / (xa[1] - xa[3]);
-            const double a3 = (a - xa[0]) / (xa[2] - xa[0]) * (a - xa[1]) / (xa[2] - xa[1]) * (a - xa[3]) / (xa[2] - xa[3]);
-            const double a4 = (a - xa[0]) / (xa[3] - xa[0]) * (a - xa[1]) / (xa[3] - xa[1]) * (a - xa[2]) / (xa[3] - xa[2]);
-            Da = a1 * v1[0] + a2 * v1[1] + a3 * v1[2] + a4 * v1[3];
-            HD = a1 * v2[0] + a2 * v2[1] + a3 * v2[2] + a4 * v2[3];
-        } else {
-            const double a1 = (a - xa[1]) / (xa[0] - xa[1]);
-            const double a2 = 1.0 - a1;
-            Da = a1 * v1[0] + a2 * v1[1];
-            HD = a1 * v2[0] + a2 * v2[1];
-        }
-    } else if (a >= g.amax) {
-        Da = g.D_0;
-        HD = g.HD_0;
-    } else {
-        Da = g.D_inf;
-        HD = g.HD_inf;
-    }
-    D = Da * a;
-}
 
 // Carlson's RD by duplication, stopping at relative spread 0.05, with the
 // fifth-order series (Numerical Recipes' rd, as the reference's
@@ -174,8 +122,8 @@ __device__ void derivs(double t, double y[6], double dy[6], EllState &s, const d
     const double d1e = a3b / (y[0] * y[1] * y[2]);
     const double d0 = (0.5 / a3b - s.Rvac) * Hi;
     const double d1_int = 1.5 * (d1e - 1.0) / a3b * Hi;
-    double D, HD;
-    dlinear(at, Dt, Ht, g, t, D, HD);
+    double D, HD, Da;
+    dlinear(at, Dt, Ht, g, t, D, HD, Da);
     const double d1_ext = 1.5 / a3b * Hi * D;
     double b[3];
     shape_b(y[0], y[1], y[2], b);
@@ -218,8 +166,8 @@ __global__ void __launch_bounds__(64) k_collapse_table(const double *__restrict_
     s.lam[0] = F / 3.0 * (1.0 - 3.0 * e + pv);
     s.F3 = F / 3.0;
     const double ab = 1.0 / (20.0 * F);
-    double t = ab, D, HD;
-    dlinear(at, Dt, Ht, g, ab, D, HD);
+    double t = ab, D, HD, Da;
+    dlinear(at, Dt, Ht, g, ab, D, HD, Da);
     s.Ha = hubble(ab, s);
     double y[6], dy[6], yt[6], dyt[6], dym[6];
 #pragma unroll

@@ -108,12 +108,18 @@ struct Timing {
     double stage_ms[NSTAGE] = {0, 0, 0, 0, 0, 0, 0};
 };
 
+// A pair of single-precision real transforms of an n^3 grid (x fastest):
+// out of place (kind 0, real n^3 <-> complex (n/2+1) n^2), or in place on
+// the padded layout (kind 1: real rows of pitch 2 (n/2+1) floats).
 struct FftPlans {
-    int n = 0;
+    int n = 0, kind = 0;
+    unsigned long long last_use = 0;
     rocfft_plan r2c = nullptr, c2r = nullptr;
     rocfft_execution_info info = nullptr;
     DevBuf work;
+    void release();
 };
+constexpr int kFftCache = 6;  // plan sets a context keeps (least recently used replaced)
 
 }  // namespace pp
 
@@ -129,11 +135,14 @@ struct pp_ctx {
     double tX1 = 0, tX2 = 0, tY1 = 0, tY2 = 0, tZ1 = 0, tZ2 = 0;
     bool have_table = false;
     pp::ShellTable shells;
-    // FFT
-    pp::FftPlans fft;
+    // FFT: plan sets by (n, kind); fft is the one fft_prepare made current
+    pp::FftPlans fft_cache[pp::kFftCache];
+    pp::FftPlans *fftp = nullptr;
+    unsigned long long fft_clock = 0;
     // scratch
     pp::DevBuf d_field_a, d_field_b, d_k0, d_k1, d_k2, d_mask, d_misc, d_peaks, d_scan, d_cub;
     pp::DevBuf d_in[8], d_pk, d_res, d_prof, d_queue;
+    pp::DevBuf d_iso[2];  // zero-padded (2n)^3 spectra of the multi-resolution tile fields
     // persistent-form tunables (pp_autotune_shell): small-peak threshold in
     // finalised shells (several peaks per wave), shells per LDS chunk of the
     // wave gather, the shell count above which a peak's Float64 post runs on
@@ -153,7 +162,8 @@ struct pp_ctx {
 };
 
 namespace pp {
-int fft_prepare(pp_ctx *ctx, int n);
+int fft_prepare(pp_ctx *ctx, int n);                       // makes the n^3 out-of-place set current
+int fft_plans(pp_ctx *ctx, int n, int kind, FftPlans **out);  // a cached set, without changing the current one
 void persist_release(pp_ctx *ctx);    // pp_persist.hip: drops the context's gather tables
 int noalloc_scratch_init(int device);  // pp_filter.hip: device address of the static scratch
 int ensure_device(pp_ctx *ctx);

@@ -49,23 +49,56 @@ int ensure_device(pp_ctx *ctx)
 
 static std::once_flag g_fft_once;
 
-int fft_prepare(pp_ctx *ctx, int n)
+void FftPlans::release()
 {
-    FftPlans &F = ctx->fft;
-    if (F.n == n && F.r2c) return PP_OK;
+    if (r2c) rocfft_plan_destroy(r2c);
+    if (c2r) rocfft_plan_destroy(c2r);
+    if (info) rocfft_execution_info_destroy(info);
+    r2c = c2r = nullptr;
+    info = nullptr;
+    work.release();
+    n = 0;
+    kind = 0;
+    last_use = 0;
+}
+
+static int fft_create(pp_ctx *ctx, FftPlans &F, int n, int kind)
+{
     std::call_once(g_fft_once, [] { rocfft_setup(); });
-    if (F.r2c) rocfft_plan_destroy(F.r2c);
-    if (F.c2r) rocfft_plan_destroy(F.c2r);
-    if (F.info) rocfft_execution_info_destroy(F.info);
-    F.r2c = F.c2r = nullptr;
-    F.info = nullptr;
-    F.n = 0;
+    F.release();
     // lengths are fastest-dimension first; the field is x fastest.
     size_t len[3] = {(size_t)n, (size_t)n, (size_t)n};
-    PP_FFT(rocfft_plan_create(&F.r2c, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                              rocfft_precision_single, 3, len, 1, nullptr));
-    PP_FFT(rocfft_plan_create(&F.c2r, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                              rocfft_precision_single, 3, len, 1, nullptr));
+    if (kind == 0) {
+        PP_FFT(rocfft_plan_create(&F.r2c, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                  rocfft_precision_single, 3, len, 1, nullptr));
+        PP_FFT(rocfft_plan_create(&F.c2r, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                  rocfft_precision_single, 3, len, 1, nullptr));
+    } else {
+        // in place: real rows padded to 2 (n/2+1) floats, the complex view
+        // (n/2+1) x n x n on the same memory
+        const size_t nk = (size_t)n / 2 + 1, pitch = 2 * nk;
+        size_t rs[3] = {1, pitch, pitch * n}, cs[3] = {1, nk, nk * n};
+        const size_t rd = pitch * n * n, cd = nk * n * n;
+        rocfft_plan_description fwd = nullptr, inv = nullptr;
+        PP_FFT(rocfft_plan_description_create(&fwd));
+        PP_FFT(rocfft_plan_description_create(&inv));
+        rocfft_status st = rocfft_plan_description_set_data_layout(
+            fwd, rocfft_array_type_real, rocfft_array_type_hermitian_interleaved, nullptr, nullptr, 3, rs, rd, 3, cs,
+            cd);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_description_set_data_layout(inv, rocfft_array_type_hermitian_interleaved,
+                                                         rocfft_array_type_real, nullptr, nullptr, 3, cs, cd, 3, rs,
+                                                         rd);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&F.r2c, rocfft_placement_inplace, rocfft_transform_type_real_forward,
+                                    rocfft_precision_single, 3, len, 1, fwd);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&F.c2r, rocfft_placement_inplace, rocfft_transform_type_real_inverse,
+                                    rocfft_precision_single, 3, len, 1, inv);
+        rocfft_plan_description_destroy(fwd);
+        rocfft_plan_description_destroy(inv);
+        PP_FFT(st);
+    }
     size_t w1 = 0, w2 = 0;
     PP_FFT(rocfft_plan_get_work_buffer_size(F.r2c, &w1));
     PP_FFT(rocfft_plan_get_work_buffer_size(F.c2r, &w2));
@@ -77,6 +110,41 @@ int fft_prepare(pp_ctx *ctx, int n)
         PP_FFT(rocfft_execution_info_set_work_buffer(F.info, F.work.ptr, F.work.bytes));
     }
     F.n = n;
+    F.kind = kind;
+    return PP_OK;
+}
+
+int fft_plans(pp_ctx *ctx, int n, int kind, FftPlans **out)
+{
+    FftPlans *hit = nullptr, *victim = &ctx->fft_cache[0];
+    for (FftPlans &F : ctx->fft_cache) {
+        if (F.r2c && F.n == n && F.kind == kind) hit = &F;
+        if (!F.r2c) {
+            if (victim->r2c) victim = &F;
+        } else if (victim->r2c && F.last_use < victim->last_use) {
+            victim = &F;
+        }
+    }
+    if (!hit) {
+        // the stream may still run transforms of the plan set being replaced
+        if (victim->r2c) PP_HIP(hipStreamSynchronize(ctx->stream));
+        int rc = fft_create(ctx, *victim, n, kind);
+        if (rc != PP_OK) {
+            victim->release();
+            return rc;
+        }
+        hit = victim;
+    }
+    hit->last_use = ++ctx->fft_clock;
+    *out = hit;
+    return PP_OK;
+}
+
+int fft_prepare(pp_ctx *ctx, int n)
+{
+    FftPlans *F = nullptr;
+    PP_CHECK(fft_plans(ctx, n, 0, &F));
+    ctx->fftp = F;
     return PP_OK;
 }
 
@@ -144,11 +212,7 @@ int pp_ctx_destroy(pp_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
-    FftPlans &F = ctx->fft;
-    if (F.r2c) rocfft_plan_destroy(F.r2c);
-    if (F.c2r) rocfft_plan_destroy(F.c2r);
-    if (F.info) rocfft_execution_info_destroy(F.info);
-    F.work.release();
+    for (FftPlans &F : ctx->fft_cache) F.release();
     DevBuf *bufs[] = {&ctx->d_table, &ctx->d_field_a, &ctx->d_field_b, &ctx->d_k0, &ctx->d_k1,
                       &ctx->d_k2, &ctx->d_mask, &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan,
                       &ctx->d_cub, &ctx->d_pk, &ctx->d_res, &ctx->d_prof, &ctx->d_queue,
@@ -159,6 +223,7 @@ int pp_ctx_destroy(pp_ctx *ctx)
     for (DevBuf *b : bufs) b->release();
     persist_release(ctx);
     for (auto &b : ctx->d_in) b.release();
+    for (auto &b : ctx->d_iso) b.release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);

@@ -25,6 +25,7 @@
 #include <mutex>
 
 #include "pp_common.h"
+#include "pp_noise.h"
 
 namespace pp {
 
@@ -65,6 +66,18 @@ __global__ void __launch_bounds__(256) k_window(const float2 *__restrict__ in, W
     const int ix = idx - hy * nk;
     const int hz = blockIdx.y;
     if (hy > half) return;
+    const int iy2 = hy == 0 ? 0 : n - hy, iz2 = hz == 0 ? 0 : n - hz;
+    const int ny = iy2 == hy ? 1 : 2, nz = iz2 == hz ? 1 : 2;
+    // the (up to) four elements are loaded before the Float64 window is
+    // evaluated, so their latency hides behind it
+    size_t el[4];
+    float2 vin[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int iy = (q & 1) ? iy2 : hy, iz = (q & 2) ? iz2 : hz;
+        el[q] = (size_t)ix + (size_t)nk * ((size_t)iy + (size_t)n * (size_t)iz);
+        if ((q & 1) < ny && (q >> 1) < nz) vin[q] = in[el[q]];
+    }
     const double kx = dk * (double)ix, ky = dk * (double)hy, kz = dk * (double)hz;
     const double k = sqrt(kx * kx + ky * ky + kz * kz);
     const double fkR = k * Rf;
@@ -74,14 +87,12 @@ __global__ void __launch_bounds__(256) k_window(const float2 *__restrict__ in, W
         wa = (float)window_value(a.wsmooth, k, fkR);
         if (b.wsmooth >= 0) wb = (float)window_value(b.wsmooth, k, fkR);
     }
-    const int iy2 = hy == 0 ? 0 : n - hy, iz2 = hz == 0 ? 0 : n - hz;
-    const int ny = iy2 == hy ? 1 : 2, nz = iz2 == hz ? 1 : 2;
-    for (int qz = 0; qz < nz; qz++) {
-        const int iz = qz ? iz2 : hz;
-        for (int qy = 0; qy < ny; qy++) {
-            const int iy = qy ? iy2 : hy;
-            const size_t e = (size_t)ix + (size_t)nk * ((size_t)iy + (size_t)n * (size_t)iz);
-            const float2 v = in[e];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if ((q & 1) >= ny || (q >> 1) >= nz) continue;
+        {
+            const size_t e = el[q];
+            const float2 v = vin[q];
             float2 r = v;
             if (scaled) { r.x = r.x * wa; r.y = r.y * wa; }
             r.x = r.x * scale;
@@ -134,7 +145,7 @@ static int launch_window(pp_ctx *ctx, const void *in, void *out, int n, double b
 // atomics, the first max_peaks ranks are kept (PeakFind.jl:70-72) and only
 // the kept peak cells are masked.
 constexpr int kStencilBlock = 256;
-constexpr int kStencilTY = kStencilBlock / 64;
+constexpr int kStencilTY = 8;   // rows of the tile: each wave takes two, 4 apart
 constexpr int kStencilKZ = 32;
 
 struct StencilGeom {
@@ -156,67 +167,90 @@ static StencilGeom stencil_geom(int n1, int n2, int n3, int nbuff)
     return g;
 }
 
+// The staged (64+2) x (8+2) neighbourhood of a plane is 660 floats, three
+// per thread at most.  Each thread's loads for plane z+3 are issued right
+// after it stages plane z+1 (two planes in flight per thread), so the HBM
+// latency hides behind the barrier and the comparisons of the planes
+// between.
 __global__ void __launch_bounds__(kStencilBlock) k_peak_bits(const float *__restrict__ d,
                                                              const int8_t *__restrict__ mask, StencilGeom g,
                                                              float fcrit, uint64_t *__restrict__ bits)
 {
-    constexpr int LX = 64 + 2, LY = kStencilTY + 2, LN = LX * LY;
+    constexpr int LX = 64 + 2, LY = kStencilTY + 2, LN = LX * LY, NQ = (LN + kStencilBlock - 1) / kStencilBlock;
     __shared__ float tile[2][LN];
     const int lane = threadIdx.x & 63, wy = threadIdx.x >> 6;
     const int x0 = g.nbuff + 64 * blockIdx.x, y0 = g.nbuff + kStencilTY * blockIdx.y;
     const int z0 = g.nbuff + kStencilKZ * blockIdx.z;
-    const int z1 = min(z0 + kStencilKZ, g.nbuff + g.in3);
-    const long long s1 = g.n1, s12 = (long long)g.n1 * g.n2;
-    const int x = x0 + lane, y = y0 + wy;
-    const bool inner = x < g.nbuff + g.in1 && y < g.nbuff + g.in2;
-    // the two elements of the staged plane each thread loads (clamped to
-    // the grid: clamped cells only neighbour columns outside the inner cube)
-    long long off[2];
-    bool has[2];
+    const int z1 = min(z0 + kStencilKZ, g.nbuff + g.in3);  // planes z0..z1-1; z0-1..z1 are read
+    const long long s12 = (long long)g.n1 * g.n2;
+    const int x = x0 + lane;
+    const int ya = y0 + wy, yb = ya + kStencilTY / 2;
+    const bool xin = x < g.nbuff + g.in1;
+    const bool ina = xin && ya < g.nbuff + g.in2, inb = xin && yb < g.nbuff + g.in2;
+    // the plane elements each thread stages (clamped to the grid: clamped
+    // cells only neighbour columns outside the inner cube)
+    int off[NQ];
+    bool has[NQ];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < NQ; q++) {
         const int e = threadIdx.x + q * kStencilBlock;
         has[q] = e < LN;
         const int ly = e / LX, lx = e - ly * LX;
         const int gx = min(x0 - 1 + lx, g.n1 - 1), gy = min(y0 - 1 + ly, g.n2 - 1);
-        off[q] = gx + gy * s1;
+        off[q] = gx + gy * g.n1;
     }
-    auto stage = [&](int z, int buf) {
+    float ra[NQ], rb[NQ];
+    auto load = [&](int z, float *r) {
         const float *p = d + (long long)z * s12;
 #pragma unroll
-        for (int q = 0; q < 2; q++)
-            if (has[q]) tile[buf][threadIdx.x + q * kStencilBlock] = p[off[q]];
+        for (int q = 0; q < NQ; q++) r[q] = has[q] ? p[off[q]] : 0.0f;
     };
-    auto max9 = [&](int buf, float &centre) {
-        const float *t = tile[buf] + wy * LX + lane;
+    auto store = [&](const float *r, int buf) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            if (has[q]) tile[buf][threadIdx.x + q * kStencilBlock] = r[q];
+    };
+    auto max9 = [&](int buf, int row, float &centre) {
+        const float *t = tile[buf] + row * LX + lane;
         centre = t[LX + 1];
         float m = fmaxf(fmaxf(t[0], t[1]), t[2]);
         m = fmaxf(m, fmaxf(fmaxf(t[LX], t[LX + 1]), t[LX + 2]));
         m = fmaxf(m, fmaxf(fmaxf(t[2 * LX], t[2 * LX + 1]), t[2 * LX + 2]));
         return m;
     };
-    float c_prev, c_cur, c_next;
-    stage(z0 - 1, 0);
-    __syncthreads();
-    float m_prev = max9(0, c_prev);
-    stage(z0, 1);
-    __syncthreads();
-    float m_cur = max9(1, c_cur);
     const long long word_row = (long long)g.words * g.in2;
-    for (int z = z0; z < z1; z++) {
-        const int buf = (z - z0) & 1;  // plane z+1 goes where plane z-1 was
-        stage(z + 1, buf);
+    float ca, cb, na_c, nb_c;
+    load(z0 - 1, ra);
+    load(z0, rb);
+    store(ra, 0);
+    load(z0 + 1, ra);
+    __syncthreads();
+    float pa = max9(0, wy, ca), pb = max9(0, wy + kStencilTY / 2, cb);
+    store(rb, 1);
+    if (z0 + 2 <= z1) load(z0 + 2, rb);
+    __syncthreads();
+    float qa = max9(1, wy, ca), qb = max9(1, wy + kStencilTY / 2, cb);
+    // plane z+1 sits in r (staged at iteration z); r then fetches plane z+3
+    auto step = [&](int z, float *r) {
+        const int buf = (z - z0) & 1;
+        store(r, buf);
+        if (z + 3 <= z1) load(z + 3, r);
         __syncthreads();
-        const float m_next = max9(buf, c_next);
-        const float m27 = fmaxf(fmaxf(m_prev, m_cur), m_next);
-        bool pk = inner && c_cur >= fcrit && !(c_cur < m27);
-        if (pk) pk = mask[x + y * s1 + (long long)z * s12] != 1;
-        const unsigned long long w = __ballot(pk);
-        if (lane == 0 && y < g.nbuff + g.in2)
-            bits[(long long)(z - g.nbuff) * word_row + (long long)(y - g.nbuff) * g.words + blockIdx.x] = w;
-        m_prev = m_cur;
-        m_cur = m_next;
-        c_cur = c_next;
+        const float ma = max9(buf, wy, na_c), mb = max9(buf, wy + kStencilTY / 2, nb_c);
+        bool ka = ina && ca >= fcrit && !(ca < fmaxf(fmaxf(pa, qa), ma));
+        bool kb = inb && cb >= fcrit && !(cb < fmaxf(fmaxf(pb, qb), mb));
+        if (ka) ka = mask[x + (long long)ya * g.n1 + (long long)z * s12] != 1;
+        if (kb) kb = mask[x + (long long)yb * g.n1 + (long long)z * s12] != 1;
+        const unsigned long long wa = __ballot(ka), wb = __ballot(kb);
+        const long long base = (long long)(z - g.nbuff) * word_row + blockIdx.x;
+        if (lane == 0 && ya < g.nbuff + g.in2) bits[base + (long long)(ya - g.nbuff) * g.words] = wa;
+        if (lane == 0 && yb < g.nbuff + g.in2) bits[base + (long long)(yb - g.nbuff) * g.words] = wb;
+        pa = qa; qa = ma; ca = na_c;
+        pb = qb; qb = mb; cb = nb_c;
+    };
+    for (int z = z0; z < z1; z += 2) {
+        step(z, ra);
+        if (z + 1 < z1) step(z + 1, rb);
     }
 }
 
@@ -441,14 +475,14 @@ static int download_peaks(pp_ctx *ctx, const PeakOut &o, const float *d_lapd, co
     out->count = written;
     if (written > 0) {
         size_t b4 = (size_t)written * 4;
-        if (out->i) PP_HIP(hipMemcpyAsync(out->i, o.i, b4, hipMemcpyDeviceToHost, ctx->stream));
-        if (out->j) PP_HIP(hipMemcpyAsync(out->j, o.j, b4, hipMemcpyDeviceToHost, ctx->stream));
-        if (out->k) PP_HIP(hipMemcpyAsync(out->k, o.k, b4, hipMemcpyDeviceToHost, ctx->stream));
-        if (out->filt) PP_HIP(hipMemcpyAsync(out->filt, o.filt, b4, hipMemcpyDeviceToHost, ctx->stream));
+        if (out->i) PP_HIP(hipMemcpyAsync(out->i, o.i, b4, hipMemcpyDefault, ctx->stream));
+        if (out->j) PP_HIP(hipMemcpyAsync(out->j, o.j, b4, hipMemcpyDefault, ctx->stream));
+        if (out->k) PP_HIP(hipMemcpyAsync(out->k, o.k, b4, hipMemcpyDefault, ctx->stream));
+        if (out->filt) PP_HIP(hipMemcpyAsync(out->filt, o.filt, b4, hipMemcpyDefault, ctx->stream));
         if (out->delta_s)
-            PP_HIP(hipMemcpyAsync(out->delta_s, o.delta_s, b4, hipMemcpyDeviceToHost, ctx->stream));
+            PP_HIP(hipMemcpyAsync(out->delta_s, o.delta_s, b4, hipMemcpyDefault, ctx->stream));
         if (out->lapd_s && d_lapd)
-            PP_HIP(hipMemcpyAsync(out->lapd_s, d_lapd, b4, hipMemcpyDeviceToHost, ctx->stream));
+            PP_HIP(hipMemcpyAsync(out->lapd_s, d_lapd, b4, hipMemcpyDefault, ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (found > out->capacity) {
@@ -497,7 +531,7 @@ int pp_rfft(pp_ctx *ctx, const float *in, void *out, int on_device, int n)
         dout = ctx->d_k0.ptr;
     }
     void *ib[1] = {din}, *ob[1] = {dout};
-    PP_FFT(rocfft_execute(ctx->fft.r2c, ib, ob, ctx->fft.info));
+    PP_FFT(rocfft_execute(ctx->fftp->r2c, ib, ob, ctx->fftp->info));
     if (!on_device) PP_HIP(hipMemcpyAsync(out, dout, cb, hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
     return PP_OK;
@@ -527,7 +561,7 @@ int pp_smooth_field(pp_ctx *ctx, const void *delta_k, float *out, int on_device,
         dst = ctx->d_field_a.as<float>();
     }
     void *ib[1] = {ctx->d_k1.ptr}, *ob[1] = {dst};
-    PP_FFT(rocfft_execute(ctx->fft.c2r, ib, ob, ctx->fft.info));
+    PP_FFT(rocfft_execute(ctx->fftp->c2r, ib, ob, ctx->fftp->info));
     if (!on_device) PP_HIP(hipMemcpyAsync(out, dst, rb, hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
     return PP_OK;
@@ -599,7 +633,7 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
     }
     {
         void *ib[1] = {(void *)d_delta}, *ob[1] = {ctx->d_k0.ptr};
-        PP_FFT(rocfft_execute(ctx->fft.r2c, ib, ob, ctx->fft.info));
+        PP_FFT(rocfft_execute(ctx->fftp->r2c, ib, ob, ctx->fftp->info));
     }
     int8_t *d_mask = ctx->d_mask.as<int8_t>();
     if (mask_inout)
@@ -618,12 +652,12 @@ int pp_find_peaks(pp_ctx *ctx, const float *delta, int on_device, int n, double 
         PP_CHECK(launch_window2(ctx, ctx->d_k0.ptr, ctx->d_k1.ptr, wsmooth, want_lapd ? ctx->d_k2.ptr : nullptr,
                                 PP_WINDOW_K2GAUSS, n, boxsize, Rf[f], inv));
         void *ib[1] = {ctx->d_k1.ptr}, *ob[1] = {ctx->d_field_a.ptr};
-        PP_FFT(rocfft_execute(ctx->fft.c2r, ib, ob, ctx->fft.info));
+        PP_FFT(rocfft_execute(ctx->fftp->c2r, ib, ob, ctx->fftp->info));
         PP_CHECK(run_stencil(ctx, ctx->d_field_a.as<float>(), d_mask, g, fcrit[f], max_peaks_per_filter, f,
                              o, d_counters));
         if (want_lapd) {
             void *ib2[1] = {ctx->d_k2.ptr}, *ob2[1] = {ctx->d_field_b.ptr};
-            PP_FFT(rocfft_execute(ctx->fft.c2r, ib2, ob2, ctx->fft.info));
+            PP_FFT(rocfft_execute(ctx->fftp->c2r, ib2, ob2, ctx->fftp->info));
             hipLaunchKernelGGL(k_sample_field, dim3(ctx->num_cu * 4), dim3(256), 0, ctx->stream,
                                ctx->d_field_b.as<float>(), n, n, (const int32_t *)o.i, (const int32_t *)o.j,
                                (const int32_t *)o.k, d_lapd, (const long long *)d_counters, o.capacity);
@@ -690,41 +724,29 @@ int pp_find_peaks_field_noalloc(pp_ctx *ctx, const float *delta_s, int8_t *mask,
 }  // extern "C"
 
 // ------------------------------------------------------------ input noise
-// fill_noise_threefry! (src/InitialConditions/RandomField.jl:46-81): cell
-// idx (1-based, x fastest) takes the first normal of counter pair
-// (idx-1) >> 1 when idx-1 is even, the second otherwise; each pair is
-// Threefry-2x64-20 of counter (pair, 0) under key (seed, 0) (Salmon et al.
-// 2011), two 53-bit uniforms in (0, 1] and Box-Muller in Float64.
+// fill_noise_threefry! (src/InitialConditions/RandomField.jl:46-81) over n3
+// cells, one thread per counter pair (threefry_normals).
 namespace pp {
-
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
 
 __global__ void k_noise_threefry(float *__restrict__ out, int64_t n3, uint64_t seed)
 {
-    constexpr int ROT[8] = {16, 42, 12, 31, 16, 32, 24, 21};
-    const uint64_t ks[3] = {seed, 0ull, 0x1BD11BDAA9FC1A22ull ^ seed};
     const int64_t npairs = (n3 + 1) / 2;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t x0 = (uint64_t)p + ks[0], x1 = ks[1];
-#pragma unroll
-        for (int blk = 0; blk < 5; blk++) {
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                x0 += x1;
-                x1 = rotl64(x1, ROT[(4 * blk + r) & 7]);
-                x1 ^= x0;
-            }
-            const int inj = blk + 1;
-            x0 += ks[inj % 3];
-            x1 += ks[(inj + 1) % 3] + (uint64_t)inj;
-        }
-        const double u1 = ((double)(x0 >> 11) + 1.0) * 1.1102230246251565e-16;
-        const double u2 = ((double)(x1 >> 11) + 1.0) * 1.1102230246251565e-16;
-        const double r = sqrt(-2.0 * log(u1));
-        const double th = 6.283185307179586 * u2;
-        out[2 * p] = (float)(r * cos(th));
-        if (2 * p + 1 < n3) out[2 * p + 1] = (float)(r * sin(th));
+        double g1, g2;
+        threefry_normals((uint64_t)p, seed, g1, g2);
+        out[2 * p] = (float)g1;
+        if (2 * p + 1 < n3) out[2 * p + 1] = (float)g2;
     }
+}
+
+int launch_noise(pp_ctx *ctx, float *out, int64_t ncells, uint64_t seed)
+{
+    if (ncells == 0) return PP_OK;
+    const int64_t npairs = (ncells + 1) / 2;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((npairs + 255) / 256, (int64_t)ctx->num_cu * 16));
+    hipLaunchKernelGGL(k_noise_threefry, dim3(grid), dim3(256), 0, ctx->stream, out, ncells, seed);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
 }
 
 }  // namespace pp
@@ -734,10 +756,5 @@ extern "C" int pp_fill_noise_threefry(pp_ctx *ctx, float *out, int64_t ncells, u
     PP_ARG(ctx && out, "NULL argument");
     PP_ARG(ncells >= 0, "negative cell count");
     PP_CHECK(pp::ensure_device(ctx));
-    if (ncells == 0) return PP_OK;
-    const int64_t npairs = (ncells + 1) / 2;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((npairs + 255) / 256, (int64_t)ctx->num_cu * 16));
-    hipLaunchKernelGGL(pp::k_noise_threefry, dim3(grid), dim3(256), 0, ctx->stream, out, ncells, seed);
-    PP_HIP(hipGetLastError());
-    return PP_OK;
+    return pp::launch_noise(ctx, out, ncells, seed);
 }

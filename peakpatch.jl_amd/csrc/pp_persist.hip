@@ -68,12 +68,6 @@ constexpr int SMALL_MAX = 64;  // largest m of the cell-based small-peak gather 
 // flight share cache lines, and each XCD (its own L2) works through its own
 // share of every m: workgroup b runs on XCD b % NXCD (round-robin dispatch).
 constexpr int NXCD = 8;
-#ifndef PP_LOCALITY
-#define PP_LOCALITY 2  // 0: one list in m order, 1: per-XCD lists, 2: + Morton order in the lists
-#endif
-#ifndef PP_GATHER_CLASS
-#define PP_GATHER_CLASS 1  // Morton order inside size classes floor(log16 m) rather than inside each m
-#endif
 
 // (delta, delta transposed, eta x, eta y) of every cell as one float4, so the
 // gathers read the four with one 16-byte load (and a 4-byte one for eta z):
@@ -81,15 +75,7 @@ constexpr int NXCD = 8;
 // With eta2, (eta z, eta2 x, eta2 y, eta2 z) go to a second float4 array
 // (q4) for the small-peak gather and the finish pass.  The block stages
 // delta(z, x, y) for 32 x 32 (x, z) in LDS (the transposed read).
-#ifndef PP_WAVE_FIRST
-#define PP_WAVE_FIRST 1  // the side stream runs the wave post before the small peaks (needs PP_WAVE_ON_SIDE)
-#endif
-#ifndef PP_PACK_NT
-#define PP_PACK_NT 1  // k_pack4 with non-temporal loads of the eta fields and stores of the packed cells
-#endif
-#ifndef PP_PACK_TILE
-#define PP_PACK_TILE 32  // x and z extent of k_pack4's transpose tile (the fields' row runs are T floats)
-#endif
+constexpr int kPackTile = 32;  // x and z extent of k_pack4's transpose tile (the fields' row runs are T floats)
 template <int T>
 __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, const float *__restrict__ ex,
                                                const float *__restrict__ ey, const float *__restrict__ ez,
@@ -111,7 +97,6 @@ __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, cons
         const int z = z0 + r, x = x0 + tx;
         if (x < n && z < n) {
             const long a = x + (long)y * n + (long)z * nn;
-#if PP_PACK_NT
             // read once and written for later kernels: streamed past the caches
             typedef float v4f __attribute__((ext_vector_type(4)));
             const v4f po = {d[a], tile[tx][r], __builtin_nontemporal_load(ex + a), __builtin_nontemporal_load(ey + a)};
@@ -121,10 +106,6 @@ __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, cons
                                 __builtin_nontemporal_load(e2y + a), __builtin_nontemporal_load(e2z + a)};
                 __builtin_nontemporal_store(pq, (v4f *)(q4 + a));
             }
-#else
-            out[a] = make_float4(d[a], tile[tx][r], ex[a], ey[a]);
-            if (q4) q4[a] = make_float4(ez[a], e2x[a], e2y[a], e2z[a]);
-#endif
         }
     }
 }
@@ -190,14 +171,13 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v)
     return x;
 }
 
-// Size class of the wave gather's order: floor(log16 m) (PP_GATHER_CLASS), or
-// m itself.  The largest class still starts first; the fields around a place
+// Size class of the wave gather's order: floor(log16 m).  The largest class
+// still starts first; the fields around a place
 // are read for all its peaks of a class together.  Measured at c3 (classes
 // m < 256 and m >= 256 there): per m 6.6 ms, log2 classes 6.0, log4 5.6,
 // log16 5.4, one class 6.4 (its largest peaks end last), m >= 1024 first 6.5.
 __host__ __device__ __forceinline__ int gather_class(int m)
 {
-    if (!PP_GATHER_CLASS) return m;
     int c = 0;
     while ((2 << c) <= m) c++;
     return c / 4;
@@ -676,31 +656,14 @@ __device__ __forceinline__ AosProfile<FULL> profile_at(ProfBuf p, int64_t rb, co
     return AosProfile<FULL>{p.e + rb * PSE, p.g + rb * PSE, p.a + rb * PSR, rad, lapd};
 }
 
-#ifndef PP_PACK_Q4
-#define PP_PACK_Q4 1  // also pack (eta z, eta2) per cell for the small-peak gather and the finish pass
-#endif
-#ifndef PP_POST_SORT
-#define PP_POST_SORT 1  // lane post of the wave-gathered peaks in (m, fcrit crossing) order
-#endif
-#ifndef PP_WAVE_ON_SIDE
-#define PP_WAVE_ON_SIDE 1  // the wave post runs on the side stream (beside the lane post)
-#endif
-static_assert(!PP_WAVE_FIRST || PP_WAVE_ON_SIDE, "PP_WAVE_FIRST puts the wave post on the side stream");
-#ifndef PP_SMALL_GRID_CU
-#define PP_SMALL_GRID_CU 32  // workgroups per CU launched for k_small (grid-stride)
-#endif
-#ifndef PP_WAVE_GRID_CU
-#define PP_WAVE_GRID_CU 16  // waves per CU launched for the wave post (grid-stride)
-#endif
-#ifndef PP_FINISH_GRID_CU
-#define PP_FINISH_GRID_CU 64  // waves per CU launched for the finish pass (strided over its lists)
-#endif
-#ifndef PP_LANE_GRID_CU
-#define PP_LANE_GRID_CU 64  // waves per CU launched for the lane posts (grid-stride; more, finer rounds balance better)
-#endif
-#ifndef PP_POST_LANE_WAVES
-#define PP_POST_LANE_WAVES 1  // minimum waves per SIMD requested for the lane post (register budget)
-#endif
+// launch grids (measured): workgroups per CU of k_small (grid-stride), waves
+// per CU of the wave post, the finish pass and the lane posts (grid-stride:
+// finer rounds balance better); the lane post asks for one wave per SIMD as
+// its register budget
+constexpr int kSmallGridCU = 32;
+constexpr int kWaveGridCU = 16;
+constexpr int kFinishGridCU = 64;
+constexpr int kLaneGridCU = 64;
 
 // The lane post's order: m descending, then the fcrit crossing descending,
 // ties in position order (a stable radix sort of these keys), so the 64
@@ -731,7 +694,7 @@ __global__ void k_iota_from(int32_t *__restrict__ o, int64_t n, int32_t first)
 // its slowest lane) is handed to the wave post through the defer list.  With
 // a list, lane t takes the peak at list[t - lo].
 template <bool FULL>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PP_POST_LANE_WAVES)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1)))
 k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
             const int64_t *__restrict__ prof_off, PostConst C, ResultView R, int has_lap, int max_steps,
             int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer, const int32_t *__restrict__ list)
@@ -1233,11 +1196,11 @@ static int morton_order(pp_ctx *ctx, const PeakParam *pps, int64_t hi, int n, Li
 int start_pack(pp_ctx *ctx, GatherFields &G)
 {
     const int n = G.n;
-    const bool pack_q4 = PP_PACK_Q4 && G.e2x != nullptr;
+    const bool pack_q4 = G.e2x != nullptr;
     PP_CHECK(ctx->d_field_b.grow((size_t)n * n * n * sizeof(float4) * (pack_q4 ? 2 : 1)));
     PP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[2], 0));
-    constexpr int T = PP_PACK_TILE;
+    constexpr int T = kPackTile;
     dim3 grid((n + T - 1) / T, (n + T - 1) / T, n), block(T, 256 / T);
     float4 *q4 = pack_q4 ? ctx->d_field_b.as<float4>() + (size_t)n * n * n : nullptr;
     hipLaunchKernelGGL(k_pack4<T>, grid, block, 0, ctx->side, G.d, G.ex, G.ey, G.ez, G.e2x, G.e2y, G.e2z,
@@ -1313,7 +1276,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // run of one group goes to list x; the finish pass's groups are the m
     // (its list keeps the caller's order), the gather's the size classes in
     // Morton order
-    const int nx = PP_LOCALITY >= 1 ? NXCD : 1;
+    const int nx = NXCD;
     int64_t xb[NXCD + 1], xbg[NXCD + 1];
     // tab rows for groups ng (group of m: grp(m)), offsets into out; fills xbo
     auto build_xtab = [&](int ng, auto grp, std::vector<int64_t> &tab, int64_t *xbo) -> int {
@@ -1348,9 +1311,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     };
     {
         PP_CHECK(build_xtab(nb, [](int m) { return m; }, st.xtab_host, xb));
-        const bool by_class = PP_LOCALITY >= 2 && PP_GATHER_CLASS;
-        PP_CHECK(build_xtab(by_class ? 32 : nb, [&](int m) { return by_class ? gather_class(m) : m; }, st.xtab_g_host,
-                            xbg));
+        PP_CHECK(build_xtab(32, [](int m) { return gather_class(m); }, st.xtab_g_host, xbg));
         if (small_lo > 0) {
             const size_t tb = st.xtab_host.size() * sizeof(int64_t), tg = st.xtab_g_host.size() * sizeof(int64_t);
             PP_CHECK(st.d_xtab.grow(tb + tg));
@@ -1362,15 +1323,11 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
                                (const int32_t *)m_sorted, (const int32_t *)nullptr, small_lo,
                                st.d_xtab.as<int64_t>(), nx, 0, W.xf);
             PP_HIP(hipGetLastError());
-            if (PP_LOCALITY >= 2) {
-                PP_CHECK(morton_order(ctx, pps, small_lo, n, W));
-                hipLaunchKernelGGL(k_xcd_order, dim3(grid1d(ctx, small_lo)), dim3(256), 0, ctx->stream,
-                                   (const int32_t *)m_sorted, (const int32_t *)W.mo, small_lo,
-                                   (const int64_t *)(st.d_xtab.as<char>() + tb), nx, by_class ? 1 : 0, W.xg);
-                PP_HIP(hipGetLastError());
-            } else {
-                W.xg = W.xf;
-            }
+            PP_CHECK(morton_order(ctx, pps, small_lo, n, W));
+            hipLaunchKernelGGL(k_xcd_order, dim3(grid1d(ctx, small_lo)), dim3(256), 0, ctx->stream,
+                               (const int32_t *)m_sorted, (const int32_t *)W.mo, small_lo,
+                               (const int64_t *)(st.d_xtab.as<char>() + tb), nx, 1, W.xg);
+            PP_HIP(hipGetLastError());
         }
     }
     // small-peak segments (equal m, descending)
@@ -1432,7 +1389,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     while ((1 << cbits) <= cmax) cbits++;
     auto cross_order = [&]() -> int {
         const int64_t nl = cross_hi - cross_lo;
-        if (!PP_POST_SORT || ctx->post_width > 1 || nl <= 64) return PP_OK;
+        if (ctx->post_width > 1 || nl <= 64) return PP_OK;
         uint32_t *ka = (uint32_t *)W.ka, *kb = (uint32_t *)W.kb;
         hipLaunchKernelGGL(k_crossing_keys, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, (const PeakParam *)pps,
                            cross_lo, cross_hi, d_rec, (const int64_t *)d_off, C.rad, cmax, cbits, ka);
@@ -1451,7 +1408,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     auto post_lane = [&](int64_t lo, int64_t hi, bool full) -> int {
         if (hi <= lo) return PP_OK;
         const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 63) / 64,
-                                                                   (int64_t)ctx->num_cu * PP_LANE_GRID_CU));
+                                                                   (int64_t)ctx->num_cu * kLaneGridCU));
         if (full) {
             hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->side, (const PeakParam *)pps, lo, hi,
                                d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0, 0x7fffffff, d_defer,
@@ -1481,7 +1438,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     auto wave_post = [&](hipStream_t s) -> int {
         PP_HIP(hipEventRecord(ctx->ev[13], s));
         if (coop_end > 0) {
-            const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * PP_WAVE_GRID_CU));
+            const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(coop_end, (int64_t)ctx->num_cu * kWaveGridCU));
             hipLaunchKernelGGL(k_post_wave, dim3(gw), dim3(64), 0, s, (const PeakParam *)pps, (int64_t)0, coop_end,
                                d_rec, (const int64_t *)d_off, C, S.R);
             PP_HIP(hipGetLastError());
@@ -1495,7 +1452,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipStreamWaitEvent(ctx->side, ctx->ev[11], 0));
         // first, so that the finish pass (which needs its statuses) can start
         // as soon as the lane post ends
-        if (PP_WAVE_FIRST) PP_CHECK(wave_post(ctx->side));
+        PP_CHECK(wave_post(ctx->side));
         if (nblk > 0) {
             SmallArgs A;
             A.F = G; A.pps = pps; A.tab = st.d_scells.as<CellSlot>();
@@ -1505,15 +1462,14 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
             for (int x = 0; x < NXCD; x++) A.xlen[x] = sxlen[x];
             A.nx = nx;
             A.prof = d_rec; A.prof_off = d_off; A.rad = C.rad;
-            int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * PP_SMALL_GRID_CU));
+            int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (int64_t)ctx->num_cu * kSmallGridCU));
             grid = (grid + nx - 1) / nx * nx;
             hipLaunchKernelGGL(k_small, dim3(grid), dim3(64), 0, ctx->side, A);
             PP_HIP(hipGetLastError());
         }
         PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
         PP_CHECK(post_lane(small_lo, np, true));
-        if (PP_WAVE_FIRST) PP_HIP(hipEventRecord(ctx->ev[14], ctx->side));
-        else if (!PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[10], ctx->side));
+        PP_HIP(hipEventRecord(ctx->ev[14], ctx->side));
         return PP_OK;
     };
     PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[12], 0));  // the packed fields
@@ -1542,12 +1498,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     PP_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     PP_CHECK(cross_order());
     PP_CHECK(side_work());  // fork after the wave gather (measured best: beside the posts, not the gather)
-    // 5. posts: the wave post on the side stream (PP_WAVE_ON_SIDE; before the
-    // small peaks' work with PP_WAVE_FIRST, else after it), beside the lane
-    // post on the main stream
-    hipStream_t wave_stream = PP_WAVE_ON_SIDE ? ctx->side : ctx->stream;
-    if (PP_WAVE_ON_SIDE) PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-    if (!PP_WAVE_FIRST) PP_CHECK(wave_post(wave_stream));
+    // 5. posts: the wave post runs on the side stream (before the small
+    // peaks' work), beside the lane post on the main stream
+    PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
     if (ctx->post_width <= 1) {
         PP_CHECK(post_lane(coop_end, small_lo, false));
     } else if (small_lo > coop_end) {
@@ -1566,9 +1519,9 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     }
     PP_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     // 6. finish for the wave-gathered peaks (after the wave post's statuses)
-    if (PP_WAVE_ON_SIDE) PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[10], 0));
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[10], 0));
     if (small_lo > 0) {
-        int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * PP_FINISH_GRID_CU));
+        int gf = (int)std::max<int64_t>(1, std::min<int64_t>(small_lo, (int64_t)ctx->num_cu * kFinishGridCU));
         gf = (gf + nx - 1) / nx * nx;
         FinishArgs A;
         A.F = G; A.pps = pps;
@@ -1583,7 +1536,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     }
     PP_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     // join: everything after this call on the context stream sees the small peaks' results
-    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[PP_WAVE_FIRST ? 14 : 10], 0));
+    PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev[14], 0));
     if (cnt) {
         // profile export: counts and Fbar per shell in the caller's order
         std::vector<float> hr((size_t)tot * PSR);
@@ -1615,13 +1568,11 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // stage times
     PP_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
     PP_HIP(hipEventSynchronize(ctx->ev[8]));
-    // main stream: 2 start | prepare | 3 | k_gather | 4 | [wave post] | 5 | lane post |
-    // 6 | finish | 7 | join | 8;  side: 11 (fork) | k_small | 9 | small post | 13 |
-    // [wave post] | 10 (the wave post runs on one of the two)
-    // with PP_WAVE_FIRST the side stream runs 11 (fork) | wave post | 10 | k_small | 9 | small post | 14
-    const int from[Timing::NSTAGE] = {2, 3, PP_WAVE_FIRST ? 10 : 11, PP_WAVE_ON_SIDE ? 13 : 4, 5, 6, 9};
-    const int to[Timing::NSTAGE] = {3, 4, 9, PP_WAVE_ON_SIDE ? 10 : 5, 6, 7,
-                                    PP_WAVE_FIRST ? 14 : (PP_WAVE_ON_SIDE ? 13 : 10)};
+    // main stream: 2 start | prepare | 3 | k_gather | 4 | 5 | lane post | 6 |
+    // finish | 7 | join | 8;  side: 11 (fork) | 13 | wave post | 10 | k_small |
+    // 9 | small post | 14
+    const int from[Timing::NSTAGE] = {2, 3, 10, 13, 5, 6, 9};
+    const int to[Timing::NSTAGE] = {3, 4, 9, 10, 6, 7, 14};
     for (int q = 0; q < Timing::NSTAGE; q++) {
         float t = 0.0f;
         (void)hipEventElapsedTime(&t, ctx->ev[from[q]], ctx->ev[to[q]]);

@@ -18,13 +18,6 @@
 
 #include <cstdint>
 
-#ifndef PP_PAIR_UNROLL
-#define PP_PAIR_UNROLL 2  // unroll of the two-shell band loop of the one-lane-per-peak posts
-#endif
-#ifndef PP_STRAIN_UNROLL
-#define PP_STRAIN_UNROLL 1  // unroll of the kernel_strain shell loop (overlaps the loads of successive shells)
-#endif
-
 namespace pp {
 
 // Profile field order (per shell): count, F, S(3), S2(3), G(3), Gf(3), SR(9, L*3+K), lapd
@@ -287,7 +280,7 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
         // Two shells per iteration halve the instructions: their weights
         // and records are adjacent (16 bytes and 2 x 24 bytes).  The shells
         // are still added in order.
-#pragma unroll PP_PAIR_UNROLL
+#pragma unroll 2  // two pairs: four shells' loads in flight per lane
         for (; m1 < b; m1 += 2) {
             double w2[2];
             __builtin_memcpy(w2, C.wtab + row.x + (m1 - row.y), sizeof w2);
@@ -298,7 +291,7 @@ __device__ Strain kernel_strain(const PV &P, const PostConst &C, int mp, int mlo
             add_shell(w2[1], G + 3, Gf + 3, Es + 6);
         }
     }
-#pragma unroll PP_STRAIN_UNROLL
+#pragma unroll 1
     for (; m1 <= b; m1 += W) {
         const double w = C.wtab[row.x + (m1 - row.y)];
         float G[3], Gf[3], Es[6];

@@ -65,6 +65,34 @@ class Growth(C.Structure):
 
 _vp, _i, _d, _f, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int64
 
+class PkTable(C.Structure):
+    _fields_ = [("npk", C.c_int), ("k", C.c_void_p), ("P", C.c_void_p)]
+
+
+class TileFields(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in ("delta", "etax", "etay", "etaz", "eta2x", "eta2y", "eta2z", "lapd")]
+
+
+class TileGeom(C.Structure):
+    _fields_ = [("n", C.c_int), ("alatt", C.c_double), ("xbx", C.c_double), ("ybx", C.c_double),
+                ("zbx", C.c_double), ("ievol", C.c_int), ("z_out", C.c_double), ("obs", C.c_double * 3),
+                ("z_max", C.c_double), ("ilpt", C.c_int), ("Om", C.c_double), ("OL", C.c_double),
+                ("round_f32", C.c_int), ("growth", Growth), ("nchi", C.c_int), ("chi_z", C.c_void_p),
+                ("chi_chi", C.c_void_p), ("chi_zmax", C.c_double)]
+
+
+class SplitGeom(C.Structure):
+    _fields_ = [("nmesh", C.c_int), ("nbuff", C.c_int), ("ntile", C.c_int), ("M", C.c_int), ("alatt", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+BATCH_FIELDS = ("i", "j", "k", "Rf", "ir2min", "ZZon", "fcrit", "x", "y", "z", "D", "v2fac", "FcRf", "d2Rf")
+
+
+class BatchOut(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in BATCH_FIELDS]
+
+
 # pp_tile_fn(user, ctx, slot, ticket, request) -> status
 TILE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_void_p)
 
@@ -103,6 +131,14 @@ SIGNATURES = {
     "pp_shell_tunables": (_i, [_vp, _vp, _vp]),
     "pp_last_shell_stages": (_i, [_vp, C.POINTER(_d)]),
     "pp_fill_noise_threefry": (_i, [_vp, _vp, _i64, C.c_uint64]),
+    "pp_make_tile_fields": (_i, [_vp, _i, _d, C.c_uint64, C.POINTER(PkTable), C.POINTER(TileFields)]),
+    "pp_coarse_create": (_i, [_vp, C.POINTER(SplitGeom), C.POINTER(PkTable), C.POINTER(C.c_void_p)]),
+    "pp_coarse_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "pp_coarse_destroy": (_i, [_vp]),
+    "pp_split_tile_fields": (_i, [_vp, _vp, _i, _i, _i, C.POINTER(TileFields)]),
+    "pp_merge_halos": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _i, _vp, _vp, C.POINTER(C.c_int64)]),
+    "pp_bookkeep_peaks": (_i, [_vp, C.POINTER(Peaks), _vp, _i, C.POINTER(TileGeom), C.POINTER(BatchOut),
+                               C.POINTER(C.c_int64)]),
     "pp_serve_create": (_i, [C.POINTER(ServeConfig), C.POINTER(_vp)]),
     "pp_serve_submit": (_i, [_vp, _vp, C.c_size_t, C.POINTER(_i64)]),
     "pp_serve_poll": (_i, [_vp, _i64, _i, C.POINTER(_i), C.POINTER(_d)]),

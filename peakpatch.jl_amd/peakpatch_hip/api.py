@@ -227,9 +227,10 @@ class Context:
 
     def find_peaks(self, delta, boxsize: float, Rf, fcrit, wsmooth: int = 1, want_lapd: bool = False,
                    nbuff: int = 1, max_peaks_per_filter: int = 2_000_000, mask=None,
-                   capacity: int | None = None) -> PeakList:
+                   capacity: int | None = None, device_out: bool = False) -> PeakList:
         """Filter bank + stencil over all filters in the given order
-        (peak_find_tile_gpu, ext/CUDAExt.jl:3068-3197)."""
+        (peak_find_tile_gpu, ext/CUDAExt.jl:3068-3197).  device_out: the
+        peak arrays are CUDA tensors (for bookkeep_peaks) instead of numpy."""
         dp, dev, keep = _ptr(delta, np.float32, "delta")
         n = delta.shape[0]
         Rf = np.ascontiguousarray(Rf, np.float64)
@@ -243,19 +244,53 @@ class Context:
         cap = capacity if capacity is not None else max(1, min(inner, len(Rf) * max_peaks_per_filter, 1 << 24))
         return self._run_peaks(lambda pk: self.lib.pp_find_peaks(
             self.h, dp, dev, n, boxsize, Rf.ctypes.data, fc.ctypes.data, len(Rf), wsmooth, int(want_lapd),
-            nbuff, max_peaks_per_filter, mp, C.byref(pk)), cap, want_lapd, dev)
+            nbuff, max_peaks_per_filter, mp, C.byref(pk)), cap, want_lapd, dev, device_out)
 
-    def _run_peaks(self, call, cap, want_lapd, on_device) -> PeakList:
-        bufs = {k: np.zeros(cap, np.int32) for k in ("i", "j", "k", "filt")}
-        bufs["delta_s"] = np.zeros(cap, np.float32)
-        bufs["lapd_s"] = np.zeros(cap, np.float32) if want_lapd else None
-        pk = L.Peaks(cap, 0, 0, *(bufs[k].ctypes.data if bufs[k] is not None else None
-                                  for k in ("i", "j", "k", "filt", "delta_s", "lapd_s")))
-        self._call(lambda: call(pk), on_device)
+    def _run_peaks(self, call, cap, want_lapd, on_device, device_out=False) -> PeakList:
+        names = ("i", "j", "k", "filt", "delta_s", "lapd_s")
+        if device_out:
+            import torch
+            d = torch.device("cuda", self.device)
+            bufs = {k: torch.empty(cap, dtype=torch.int32, device=d) for k in names[:4]}
+            bufs["delta_s"] = torch.empty(cap, dtype=torch.float32, device=d)
+            bufs["lapd_s"] = torch.empty(cap, dtype=torch.float32, device=d) if want_lapd else None
+            addr = lambda b: b.data_ptr()
+        else:
+            bufs = {k: np.zeros(cap, np.int32) for k in names[:4]}
+            bufs["delta_s"] = np.zeros(cap, np.float32)
+            bufs["lapd_s"] = np.zeros(cap, np.float32) if want_lapd else None
+            addr = lambda b: b.ctypes.data
+        pk = L.Peaks(cap, 0, 0, *(addr(bufs[k]) if bufs[k] is not None else None for k in names))
+        self._call(lambda: call(pk), on_device or device_out)
         c = pk.count
-        return PeakList(bufs["i"][:c].copy(), bufs["j"][:c].copy(), bufs["k"][:c].copy(),
-                        bufs["filt"][:c].copy(), bufs["delta_s"][:c].copy(),
-                        None if bufs["lapd_s"] is None else bufs["lapd_s"][:c].copy())
+        cut = (lambda b: b[:c]) if device_out else (lambda b: b[:c].copy())
+        return PeakList(*(cut(bufs[k]) if bufs[k] is not None else None for k in names))
+
+    def bookkeep_peaks(self, peaks: PeakList, Rf_filters, geom: L.TileGeom) -> dict:
+        """The drivers' per-peak loop (src/Pipeline.jl:217-231, 257-271;
+        src/MultiResolution.jl:959-988) on the device, over a device PeakList
+        of find_peaks(device_out=True).  Returns the kept rows as CUDA tensors
+        keyed like L.BATCH_FIELDS (Rf, ir2min, ZZon, fcrit, x, y, z, D, v2fac
+        Float64; FcRf, d2Rf Float32; i, j, k int32)."""
+        import torch
+        n = len(peaks)
+        d = torch.device("cuda", self.device)
+        dt = {"i": torch.int32, "j": torch.int32, "k": torch.int32, "ir2min": torch.int64,
+              "FcRf": torch.float32, "d2Rf": torch.float32}
+        out = {f: torch.empty(max(n, 1), dtype=dt.get(f, torch.float64), device=d) for f in L.BATCH_FIELDS}
+        if n == 0:
+            return {f: t[:0] for f, t in out.items()}
+        for a in (peaks.i, peaks.j, peaks.k, peaks.filt, peaks.delta_s):
+            if not (_is_torch(a) and a.is_cuda):
+                raise ValueError("bookkeep_peaks needs the device PeakList of find_peaks(device_out=True)")
+        pk = L.Peaks(n, n, n, *(a.data_ptr() if a is not None else None
+                                for a in (peaks.i, peaks.j, peaks.k, peaks.filt, peaks.delta_s, peaks.lapd_s)))
+        rf = np.ascontiguousarray(Rf_filters, np.float64)
+        bo = L.BatchOut(*(out[f].data_ptr() for f in L.BATCH_FIELDS))
+        kept = C.c_int64(0)
+        self._call(lambda: self.lib.pp_bookkeep_peaks(self.h, C.byref(pk), rf.ctypes.data, len(rf), C.byref(geom),
+                                                       C.byref(bo), C.byref(kept)), True)
+        return {f: t[:kept.value] for f, t in out.items()}
 
     # ----------------------------------------------------- shell analysis
     def _fields(self, delta, eta, eta2=None, lapd=None):

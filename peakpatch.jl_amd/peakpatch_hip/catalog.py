@@ -45,6 +45,40 @@ def pack_halos(ctx, res: dict, bdev: dict, alatt: float, extended: bool, z_max: 
     return out[: nh.value], float(rmax.value)
 
 
+def merge_catalog(ctx, records):
+    """merge_catalog (src/Merger/Merger.jl:21-79) on the device
+    (pp_merge_halos): Lagrangian exclusion and volume reduction of a
+    catalogue of HaloRecord / ExtHaloRecord rows (numpy (nhalo, 11 | 33) or a
+    CUDA tensor).  Returns the surviving rows in catalogue order with RTHL
+    replaced by the reduced radius, of the same kind as the input."""
+    on_dev = type(records).__module__.startswith("torch")
+    n = int(records.shape[0])
+    if n == 0:
+        return records[:0]
+    nk = C.c_int64()
+    if on_dev:
+        import torch
+        cols = [records[:, q].contiguous() for q in (0, 1, 2, 6)]
+        keep = torch.empty(n, dtype=torch.int8, device=records.device)
+        nr = torch.empty(n, dtype=torch.float32, device=records.device)
+        ctx._call(lambda: ctx.lib.pp_merge_halos(ctx.h, n, *(c.data_ptr() for c in cols), 1, keep.data_ptr(),
+                                                 nr.data_ptr(), C.byref(nk)), True)
+        sel = keep.bool()
+        out = records[sel].clone()
+        out[:, 6] = nr[sel]
+        return out
+    rec = np.ascontiguousarray(records, np.float32)
+    cols = [np.ascontiguousarray(rec[:, q]) for q in (0, 1, 2, 6)]
+    keep = np.zeros(n, np.int8)
+    nr = np.zeros(n, np.float32)
+    L.check(ctx.lib.pp_merge_halos(ctx.h, n, *(c.ctypes.data for c in cols), 0, keep.ctypes.data, nr.ctypes.data,
+                                   C.byref(nk)))
+    sel = keep.astype(bool)
+    out = rec[sel].copy()
+    out[:, 6] = nr[sel]
+    return out
+
+
 def write_pksc(path: str, records: np.ndarray, RTHLmax: float, z_out: float) -> None:
     """write_pksc (Catalog.jl:53-94)."""
     rec = np.ascontiguousarray(records, dtype="<f4")

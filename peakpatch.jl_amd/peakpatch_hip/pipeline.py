@@ -14,6 +14,7 @@ every peak's shell sphere stays inside the tile.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass, field, replace
 
@@ -157,13 +158,15 @@ class TileDriver:
             f = float(self.growth.fsc_of_z(c.z_out))
         return np.full(c.nfilt, np.float32(f), np.float32)
 
-    def find_peaks(self, ctx, delta, mask=None):
+    def find_peaks(self, ctx, delta, mask=None, device_out=False):
         c = self.cfg
         return ctx.find_peaks(delta, c.boxsize, c.filters, self.filter_fcrit(), wsmooth=c.wsmooth,
-                              want_lapd=c.ioutshear >= 1, nbuff=c.nbuff_eff, mask=mask)
+                              want_lapd=c.ioutshear >= 1, nbuff=c.nbuff_eff, mask=mask, device_out=device_out)
 
-    def batch(self, peaks) -> PeakBatch:
-        """Per-peak bookkeeping (Pipeline.jl:217-231, MultiResolution.jl:959-988)."""
+    def batch(self, peaks, round_f32: bool = False) -> PeakBatch:
+        """Per-peak bookkeeping (Pipeline.jl:217-231, MultiResolution.jl:959-988).
+        round_f32: ZZon and fcrit pass through Float32 as run_multitile_split
+        stores them (MultiResolution.jl:963-967); run_tile keeps Float64."""
         c = self.cfg
         Rf = c.filters[peaks.filt]
         cen = 0.5 * (c.n + 1)
@@ -184,12 +187,41 @@ class TileDriver:
             v2 = -(-3.0 / 7.0 * Om_a ** (-1.0 / 143) * D_pk ** 2)
         else:
             v2 = np.zeros(len(x))
-        b = PeakBatch(peaks.i, peaks.j, peaks.k, Rf, cosmo_mod.ir2min(Rf, c.alatt), 1.0 + zpk,
-                      self.growth.fsc_of_z(zpk), x, y, z, peaks.delta_s,
+        ZZon, fcrit = 1.0 + zpk, self.growth.fsc_of_z(zpk)
+        if round_f32:
+            ZZon = ZZon.astype(np.float32).astype(np.float64)
+            fcrit = fcrit.astype(np.float32).astype(np.float64)
+        b = PeakBatch(peaks.i, peaks.j, peaks.k, Rf, cosmo_mod.ir2min(Rf, c.alatt), ZZon,
+                      fcrit, x, y, z, peaks.delta_s,
                       peaks.lapd_s if peaks.lapd_s is not None else np.zeros(len(x), np.float32), D_pk, v2)
         if c.ievol == 1:
             b = b.take(zpk <= c.z_max)
         return b
+
+    def geom(self, round_f32: bool = False):
+        """The tile geometry and host tables pp_bookkeep_peaks reads; the
+        numpy tables are kept alive on the driver."""
+        from . import _lib as L
+        c = self.cfg
+        if getattr(self, "_geom", None) is None or self._geom[0] != round_f32:
+            g = self.growth
+            cols = [np.ascontiguousarray(a, np.float64) for a in (g.atab, g.Datab, g.dlnD)]
+            chi = ([np.ascontiguousarray(self.chi2z.zs, np.float64), np.ascontiguousarray(self.chi2z.chis, np.float64)]
+                   if c.ievol == 1 else [])
+            gr = L.Growth(len(cols[0]), *(a.ctypes.data for a in cols))
+            geo = L.TileGeom(c.n, c.alatt, *map(float, c.tile_center), int(c.ievol), float(c.z_out),
+                             (C.c_double * 3)(*map(float, c.cen)), float(c.z_max), int(c.ilpt),
+                             float(c.cosmology.Om), float(c.cosmology.OL), int(round_f32), gr,
+                             len(chi[0]) if chi else 0, chi[0].ctypes.data if chi else None,
+                             chi[1].ctypes.data if chi else None, float(self.chi2z.z_max) if chi else 0.0)
+            self._geom = (round_f32, geo, cols, chi)
+        return self._geom[1]
+
+    def device_batch(self, ctx, peaks, round_f32: bool = False) -> dict:
+        """batch() on the device: peaks is the device PeakList of
+        find_peaks(..., device_out=True); returns PeakBatch.to_device()'s
+        dict without a host round trip."""
+        return ctx.bookkeep_peaks(peaks, self.cfg.filters, self.geom(round_f32))
 
     def analyse(self, ctx, fields, b, mask=None, kernel=SHELL_PERSISTENT, device_out=False):
         """b: a PeakBatch (host) or the dict of CUDA tensors of PeakBatch.to_device()."""

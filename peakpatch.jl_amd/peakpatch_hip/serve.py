@@ -7,8 +7,8 @@ Design
 * Slots.  Each slot owns a Context (its own HIP stream, pooled device
   buffers, the collapse table and the shell tables, all resident for the
   server's life) and one worker thread.  Everything a tile does -- input
-  synthesis (torch on the slot's stream), filter bank + stencil, shell
-  analysis, record packing -- is queued on that one stream, so tiles on
+  synthesis, filter bank + stencil, shell analysis, record packing -- is
+  queued on that one stream, so tiles on
   different slots overlap on the device while each tile stays in order.
 * Admission by free HBM.  A tile's device footprint is estimated from its
   nmesh (``footprint_bytes``).  The scheduler admits the oldest waiting tile
@@ -298,11 +298,16 @@ def summarize(results) -> dict:
 class GpuTileRunner:
     """The per-tile GPU pipeline on one slot: synthesis -> filter bank +
     stencil -> per-peak bookkeeping -> shell analysis -> record packing, all
-    on the slot context's stream."""
+    on the slot context's stream; the peak list and the per-peak batch stay
+    on the device."""
 
-    def __init__(self, device: int = 0, template: TileConfig = SERVE_TEMPLATE, table=None, tunables=None):
+    def __init__(self, device: int = 0, template: TileConfig = SERVE_TEMPLATE, table=None, tunables=None,
+                 round_f32: bool = True):
         from . import collapse
         self.device = device
+        # run_multitile_split stores per-peak ZZon and fcrit as Float32
+        # (MultiResolution.jl:963-967)
+        self.round_f32 = round_f32
         self.template = template
         self.table = table if table is not None else collapse.sinusoidal_table()
         self.tunables = tunables
@@ -344,18 +349,16 @@ class GpuTileRunner:
             t[0] = now
 
         with torch.cuda.stream(stream):
-            tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt,
-                                         want_lapd=cfg.ioutshear >= 1, device=f"cuda:{self.device}", ctx=ctx)
+            tile = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
             lap("synthesis")
-            peaks = drv.find_peaks(ctx, tile.delta)
+            peaks = drv.find_peaks(ctx, tile.delta, device_out=True)
             lap("peak_find")
-            b = drv.batch(peaks)
+            bdev = drv.device_batch(ctx, peaks, round_f32=self.round_f32 and cfg.ievol == 1)
             lap("bookkeeping")
-            out = TileResult(req.tile_id, req.nmesh, npeaks=len(b), stages=st)
-            if len(b) == 0:
+            out = TileResult(req.tile_id, req.nmesh, npeaks=len(bdev["i"]), stages=st)
+            if out.npeaks == 0:
                 out.records = np.zeros((0, 33), np.float32)
                 return out
-            bdev = b.to_device(f"cuda:{self.device}")
             mask = torch.zeros((cfg.n,) * 3, dtype=torch.int8, device=f"cuda:{self.device}")
             fields = (tile.delta, tile.eta, tile.eta2, tile.lapd)
             res = drv.analyse(ctx, fields, bdev, mask=mask, kernel=SHELL_PERSISTENT, device_out=True)

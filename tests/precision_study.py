@@ -86,7 +86,7 @@ def study(cfg_name: str = "c4", n: int | None = None, fp16: bool = True, threads
     ctx.upload_collapse_table(ct.table, *ct.bounds)
     ctx.build_shell_tables(cfg.nhunt)
     t0 = time.perf_counter()
-    tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
+    tile = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
     drv = TileDriver(cfg)
     peaks = drv.find_peaks(ctx, tile.delta)
     b = drv.batch(peaks)

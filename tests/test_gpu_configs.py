@@ -44,8 +44,7 @@ def _table():
 def _tile(ctx, cfg):
     import torch
     from peakpatch_hip import synth
-    t = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1,
-                              device="cuda", ctx=ctx)
+    t = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
     torch.cuda.synchronize()
     return t
 

@@ -3,8 +3,15 @@ import numpy as np
 import pytest
 
 from conftest import ipp1
+from oracle import fields as OF
 from oracle import oracle as O
 from peakpatch_hip import synth
+
+
+def _cpu_tile(n, box, seed, ilpt=1, want_lapd=False):
+    """Small tile fields from the oracle's CPU restatement of run_tile's
+    input generation (host arrays; delta_k included)."""
+    return OF.make_tile(n, box, seed, OF.PkTable(*synth.pk_table()), ilpt=ilpt, want_lapd=want_lapd)
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +41,7 @@ def test_window_multiply_vs_oracle(ctx, wsmooth, n, box, Rf):
 @pytest.mark.parametrize("wsmooth", [0, 1, 3])
 def test_smooth_field_vs_fp64(ctx, wsmooth):
     n, box, Rf = 64, 75.0, 2.5
-    t = synth.make_tile(n, box, seed=7, ilpt=1, want_lapd=False)
+    t = _cpu_tile(n, box, 7)
     g = ctx.smooth_field(t.delta_k, n, box, Rf, wsmooth)
     o = O.smooth_field(t.delta_k, n, box, Rf, wsmooth)
     scale = np.abs(o).max()
@@ -125,7 +132,7 @@ def test_filter_bank_vs_oracle(ctx):
     fcrit per filter; peak lists compared with the oracle run on the same
     delta_k (FFTW/pocketfft vs rocFFT last bits can only flip exact ties)."""
     n, box, nbuff = 64, 75.0, 4
-    t = synth.make_tile(n, box, seed=42, ilpt=1, want_lapd=False)
+    t = _cpu_tile(n, box, 42)
     Rf = synth.filter_bank(6, 1.5, 8.0)
     fcrit = 1.686
     mask_g = np.zeros((n, n, n), np.int8)
@@ -162,7 +169,7 @@ def test_threefry_noise_kernel_matches_host(ctx):
     import torch
     from peakpatch_hip import synth
     n = 33
-    ref = synth.fill_noise_threefry(n, 1234)
+    ref = OF.fill_noise_threefry(n, 1234)
     got = ctx.fill_noise(n, 1234)
     ctx.synchronize()
     got = got.cpu().numpy()
@@ -187,21 +194,23 @@ def test_fill_noise_is_ordered_before_torch_work(ctx):
         assert torch.equal(got.cpu(), ref)
 
 
-def test_tile_synthesis_with_context_is_deterministic(ctx):
-    """make_tile_torch(ctx=...) with no synchronisation between the noise
-    kernel and torch's FFTs gives the same fields as a synchronised draw."""
+def test_tile_fields_match_the_cpu_restatement(ctx):
+    """pp_make_tile_fields against the oracle's restatement of run_tile's
+    input generation (generate_grf, 1LPT, 2LPT, k^2 field) on the same seed
+    and P(k) table: rocFFT Float32 against pocketfft Float64 transforms, so
+    the fields agree to Float32 transform noise; two device runs are bit
+    identical."""
     import torch
-    from peakpatch_hip import synth
-    n, box = 128, 100.0
-    ref_noise = ctx.fill_noise(n, 7)
-    ctx.synchronize()
+    n, box, seed = 64, 100.0, 7
+    t1 = synth.make_tile(ctx, n, box, seed, ilpt=2, want_lapd=True)
+    t2 = synth.make_tile(ctx, n, box, seed, ilpt=2, want_lapd=True)
     torch.cuda.synchronize()
-    t_ref = synth.make_tile_torch(n, box, 7, ilpt=1, want_lapd=False, device="cuda")
-    t_ctx = synth.make_tile_torch(n, box, 7, ilpt=1, want_lapd=False, device="cuda", ctx=ctx)
-    t_ctx2 = synth.make_tile_torch(n, box, 7, ilpt=1, want_lapd=False, device="cuda", ctx=ctx)
-    torch.cuda.synchronize()
-    assert torch.equal(t_ctx.delta, t_ctx2.delta)
-    # the torch Threefry restatement and the kernel draw the same noise up
-    # to the last bits of Box-Muller, so the fields agree to float32 noise
-    assert torch.allclose(t_ctx.delta, t_ref.delta, rtol=0, atol=1e-4 * float(t_ref.delta.abs().max()))
-    assert ref_noise.shape == (n, n, n)
+    ref = _cpu_tile(n, box, seed, ilpt=2, want_lapd=True)
+    pairs = [("delta", t1.delta, t2.delta, ref.delta), ("lapd", t1.lapd, t2.lapd, ref.lapd)]
+    pairs += [(f"eta{q}", a, b, c) for q, (a, b, c) in enumerate(zip(t1.eta, t2.eta, ref.eta))]
+    pairs += [(f"eta2{q}", a, b, c) for q, (a, b, c) in enumerate(zip(t1.eta2, t2.eta2, ref.eta2))]
+    for name, a, b, c in pairs:
+        assert torch.equal(a, b), name
+        g = a.cpu().numpy()
+        scale = float(np.abs(c).max())
+        assert np.abs(g - c).max() <= 2e-5 * scale, (name, float(np.abs(g - c).max()), scale)

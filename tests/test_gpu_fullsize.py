@@ -40,8 +40,7 @@ def test_c3_full_tile():
     try:
         ctx.upload_collapse_table(ct.table, *ct.bounds)
         ctx.build_shell_tables(cfg.nhunt)
-        tile = synth.make_tile_torch(cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1,
-                                     device="cuda", ctx=ctx)
+        tile = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
         ctx.synchronize()
         drv = TileDriver(cfg)
         b = drv.batch(drv.find_peaks(ctx, tile.delta))

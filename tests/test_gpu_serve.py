@@ -1,6 +1,7 @@
 """The tile-serving loop on the GPU: every tile's catalogue is the same
 alone and in a mix of concurrent tiles (continuous batching over several
-slots, work overlapping on the device)."""
+slots, work overlapping on the device), with the native server's own slot
+contexts and its admission by device memory."""
 import numpy as np
 import pytest
 
@@ -22,19 +23,18 @@ def test_tiles_same_alone_and_in_mix(homeltab_path):
     from peakpatch_hip import collapse
     from peakpatch_hip.serve import GpuTileRunner
     runner = GpuTileRunner(table=collapse.read_homeltab(homeltab_path), tunables=TUNE)
+    # the server's own slot contexts, admission by device memory (budget:
+    # a fraction of the free HBM, and the live free memory per tile)
     # alone: one slot, each tile submitted after the previous one finished
-    st = [runner.make_slot()]
-    with TileServer(slots=1, run_tile=runner.run, slot_states=st) as srv:
+    with TileServer(slots=1, runner=runner) as srv:
+        assert srv.budget > 0
         alone = [srv.submit(r).result(timeout=300) for r in _requests()]
-    runner.release_slot(st[0])
     # mix: three slots, all tiles arriving together
-    st = [runner.make_slot() for _ in range(3)]
-    with TileServer(slots=3, run_tile=runner.run, slot_states=st) as srv:
+    with TileServer(slots=3, runner=runner) as srv:
         mix = replay(srv, [(0.0, r) for r in _requests()])
         overlap = srv.max_in_flight
-    for s in st:
-        runner.release_slot(s)
-    assert overlap == 3
+        st = srv.stats()
+    assert overlap == 3 and st["finished"] == 6 and st["failed"] == 0
     assert all(r.error is None for r in alone + mix), [r.error for r in alone + mix]
     assert sum(r.nhalo for r in alone) > 50
     for a, m in zip(alone, mix):

@@ -54,15 +54,30 @@ def test_ir2min():
 def test_threefry_known_answer_and_moments():
     """Threefry-2x64-20: Random123 known-answer vector (zero counter, zero
     key); the Box-Muller normals have unit variance."""
-    x0, x1 = synth.threefry2x64_20(np.uint64(0), np.uint64(0), np.uint64(0), np.uint64(0))
+    from oracle import fields as OF
+    x0, x1 = OF.threefry2x64_20(np.uint64(0), np.uint64(0), np.uint64(0), np.uint64(0))
     assert (int(x0), int(x1)) == (0xC2B6E3A8C2C69865, 0x6F81ED42F350084D)
-    g1, g2 = synth.threefry_gaussian(42, np.arange(200000, dtype=np.uint64))
+    g1, g2 = OF.threefry_gaussian(42, np.arange(200000, dtype=np.uint64))
     g = np.concatenate([g1, g2])
     assert abs(g.mean()) < 0.01 and abs(g.var() - 1.0) < 0.01
     # counter mode: the same seed and index give the same number
-    a = synth.fill_noise_threefry(8, 7)
-    b = synth.fill_noise_threefry(8, 7)
-    assert np.array_equal(a, b) and not np.array_equal(a, synth.fill_noise_threefry(8, 8))
+    a = OF.fill_noise_threefry(8, 7)
+    b = OF.fill_noise_threefry(8, 7)
+    assert np.array_equal(a, b) and not np.array_equal(a, OF.fill_noise_threefry(8, 8))
+
+
+def test_pk_table_interpolation():
+    """load_pk's log-log interpolation (PowerSpectrum.jl:11-23) of the EH98
+    table reproduces the closed form between rows and extrapolates linearly
+    in log-log beyond them."""
+    from oracle import fields as OF
+    k, P = synth.pk_table()
+    f = OF.PkTable(k, P)
+    kk = np.exp(np.linspace(np.log(2e-4), np.log(50.0), 777))
+    assert np.allclose(f(kk), synth.EH98NoWiggle()(kk), rtol=2e-5)
+    lo = f(np.array([k[0] / 10]))[0]
+    slope = (np.log(P[1]) - np.log(P[0])) / (np.log(k[1]) - np.log(k[0]))
+    assert lo == pytest.approx(P[0] * 10.0 ** -slope, rel=1e-9)
 
 
 def test_power_spectrum_normalisation():
