@@ -147,21 +147,23 @@ def read_traffic(path, head, kernel):
 
 
 def serve_stream(args, ct, tuned):
-    """The c5 serving workload: light-cone tiles with nmesh drawn from
-    {128, 256, 384, 512, 768} at the c2 resolution, each synthesised,
-    peak-found, bookkept, shell-analysed and packed on the GPU, on the native
-    server's slot contexts with admission by device memory.  Two runs of
+    """The c5 serving workload: run_multitile_split light-cone tiles
+    (coarse_factor 5) with nmesh drawn from {128, 256, 384, 512, 768} at the c2
+    resolution, each generated from its nmesh's resident coarse field plus its
+    own isolated residual, peak-found, bookkept, shell-analysed and packed on
+    the GPU, on the native server's slot contexts with admission by device
+    memory.  Two runs of
     args.serve_tiles tiles: all submitted at once (the drain rate is the
     saturation capacity), then Poisson arrivals at args.serve_load times that
     capacity (or at args.serve_rate); latency = completion - arrival,
     queueing included."""
     from peakpatch_hip import serve
-    runner = serve.GpuTileRunner(table=ct, tunables=tuned)
+    runner = serve.GpuTileRunner(table=ct, tunables=tuned, split=True)
     n = args.serve_tiles
     with serve.TileServer(slots=args.serve_slots, runner=runner, hbm_fraction=0.8) as srv:
-        # warm the slots on every nmesh (transform plans, buffer pools), untimed
-        warm = [serve.TileRequest(-1, m, (400.0, 400.0, 400.0), seed=1)
-                for m in serve.SERVE_NMESH for _ in range(args.serve_slots)]
+        # warm the slots on every nmesh (the resident coarse fields, transform
+        # plans, buffer pools), untimed
+        warm = [serve.TileRequest(-1, m, tile=(1, 1, 1)) for m in serve.SERVE_NMESH for _ in range(args.serve_slots)]
         bad = [r.error for r in serve.replay(srv, [(0.0, r) for r in warm]) if r.error]
         if bad:
             raise RuntimeError(f"serving warm-up failed: {bad[0]}")
@@ -171,10 +173,16 @@ def serve_stream(args, ct, tuned):
         stream = serve.poisson_stream(n, rate, seed=6)
         res = serve.replay(srv, stream)
         st = srv.stats()
+    geoms = {m: runner.split_geometry(m) for m in serve.SERVE_NMESH}
+    runner.close()
     s = serve.summarize(res)
     mesh = [r.nmesh for _, r in stream]
-    s.update({"workload": f"c5: {n} tiles, Poisson {rate:.3g}/s, nmesh from {serve.SERVE_NMESH}, alatt 250/256 "
-                          f"Mpc/h, 20 filters 1-15 Mpc/h, 2LPT, light-cone",
+    s.update({"workload": f"c5: {n} run_multitile_split tiles (coarse_factor {serve.COARSE_FACTOR}, "
+                          f"{serve.SPLIT_NTILE}^3 tiles per nmesh, coarse fields resident), Poisson {rate:.3g}/s, "
+                          f"nmesh from {serve.SERVE_NMESH}, alatt 250/256 Mpc/h, 20 filters 1-15 Mpc/h, 2LPT, "
+                          f"light-cone",
+              "split_geometry": {str(m): {"N": g.N, "M": g.M, "block": g.block, "nbuff": g.nbuff}
+                                 for m, g in geoms.items()},
               "arrival_rate_per_s": rate,
               "load": rate / cap["tiles_per_s"] if cap["tiles_per_s"] > 0 else None,
               "capacity_tiles_per_s": cap["tiles_per_s"],

@@ -114,6 +114,7 @@ struct Timing {
 struct FftPlans {
     int n = 0, kind = 0;
     unsigned long long last_use = 0;
+    size_t work_need = 0;  // work area bytes; kind 1 sets its work area per use (stream ordered)
     rocfft_plan r2c = nullptr, c2r = nullptr;
     rocfft_execution_info info = nullptr;
     DevBuf work;
@@ -142,7 +143,6 @@ struct pp_ctx {
     // scratch
     pp::DevBuf d_field_a, d_field_b, d_k0, d_k1, d_k2, d_mask, d_misc, d_peaks, d_scan, d_cub;
     pp::DevBuf d_in[8], d_pk, d_res, d_prof, d_queue;
-    pp::DevBuf d_iso[2];  // zero-padded (2n)^3 spectra of the multi-resolution tile fields
     // persistent-form tunables (pp_autotune_shell): small-peak threshold in
     // finalised shells (several peaks per wave), shells per LDS chunk of the
     // wave gather, the shell count above which a peak's Float64 post runs on

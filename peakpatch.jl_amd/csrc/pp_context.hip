@@ -60,6 +60,7 @@ void FftPlans::release()
     n = 0;
     kind = 0;
     last_use = 0;
+    work_need = 0;
 }
 
 static int fft_create(pp_ctx *ctx, FftPlans &F, int n, int kind)
@@ -105,7 +106,8 @@ static int fft_create(pp_ctx *ctx, FftPlans &F, int n, int kind)
     size_t w = std::max(w1, w2);
     PP_FFT(rocfft_execution_info_create(&F.info));
     PP_FFT(rocfft_execution_info_set_stream(F.info, ctx->stream));
-    if (w > 0) {
+    F.work_need = w;
+    if (w > 0 && kind == 0) {
         PP_CHECK(F.work.grow(w));
         PP_FFT(rocfft_execution_info_set_work_buffer(F.info, F.work.ptr, F.work.bytes));
     }
@@ -223,7 +225,6 @@ int pp_ctx_destroy(pp_ctx *ctx)
     for (DevBuf *b : bufs) b->release();
     persist_release(ctx);
     for (auto &b : ctx->d_in) b.release();
-    for (auto &b : ctx->d_iso) b.release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
@@ -277,6 +278,23 @@ int pp_ctx_signal_stream(pp_ctx *ctx, void *stream)
     PP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
     PP_HIP(hipEventRecord(ctx->ev_join, ctx->stream));
     PP_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+    return PP_OK;
+}
+
+int pp_ctx_release_scratch(pp_ctx *ctx)
+{
+    PP_ARG(ctx, "NULL context");
+    PP_CHECK(ensure_device(ctx));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->side));
+    for (FftPlans &F : ctx->fft_cache) F.release();
+    ctx->fftp = nullptr;
+    DevBuf *bufs[] = {&ctx->d_field_a, &ctx->d_field_b, &ctx->d_k0, &ctx->d_k1, &ctx->d_k2, &ctx->d_mask,
+                      &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan, &ctx->d_cub, &ctx->d_pk, &ctx->d_res,
+                      &ctx->d_prof, &ctx->d_queue};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &b : ctx->d_in) b.release();
+    persist_release(ctx);
     return PP_OK;
 }
 

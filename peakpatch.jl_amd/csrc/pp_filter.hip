@@ -130,9 +130,9 @@ static int launch_window(pp_ctx *ctx, const void *in, void *out, int n, double b
 }
 
 // ------------------------------------------------------------- stencil
-// One pass over the field per filter.  A workgroup takes a 64 x 4 tile of
+// One pass over the field per filter.  A workgroup takes a 64 x 16 tile of
 // inner (x, y) columns and marches along z through kStencilKZ planes: each
-// plane's (64+2) x (4+2) neighbourhood is staged in LDS, every column keeps
+// plane's (64+2) x (16+2) neighbourhood is staged in LDS, every column keeps
 // the 3x3 (x, y) maxima of planes z-1, z, z+1 in registers, and the strict
 // 27-point test "no neighbour above ff" is ff >= max of the 27 values.  A
 // wave is one x-run of 64 cells of one row, so its ballot of accepted cells
@@ -144,9 +144,20 @@ static int launch_window(pp_ctx *ctx, const void *in, void *out, int n, double b
 // (k outer, j, i inner), so each set bit is written at its rank with no
 // atomics, the first max_peaks ranks are kept (PeakFind.jl:70-72) and only
 // the kept peak cells are masked.
-constexpr int kStencilBlock = 256;
-constexpr int kStencilTY = 8;   // rows of the tile: each wave takes two, 4 apart
+//
+// Tiles are numbered x fastest, then y, then z, and handed out so that each
+// XCD (workgroups go round-robin to the eight XCDs, each with its own L2)
+// works through one contiguous run of them: neighbouring tiles share their
+// halo rows and planes, and those lines are then read from HBM once per run
+// instead of once per tile.
+constexpr int kStencilBlock = 512;
+constexpr int kStencilTY = 16;  // rows of the tile: each wave takes two, 8 apart
 constexpr int kStencilKZ = 32;
+constexpr int kStencilXCD = 8;
+
+struct StencilTiles {
+    int nbx, nby, nbz, per_xcd;  // tiles along x, y, z; tiles per XCD run
+};
 
 struct StencilGeom {
     int n1, n2, n3, nbuff;
@@ -174,13 +185,16 @@ static StencilGeom stencil_geom(int n1, int n2, int n3, int nbuff)
 // between.
 __global__ void __launch_bounds__(kStencilBlock) k_peak_bits(const float *__restrict__ d,
                                                              const int8_t *__restrict__ mask, StencilGeom g,
-                                                             float fcrit, uint64_t *__restrict__ bits)
+                                                             StencilTiles tl, float fcrit, uint64_t *__restrict__ bits)
 {
     constexpr int LX = 64 + 2, LY = kStencilTY + 2, LN = LX * LY, NQ = (LN + kStencilBlock - 1) / kStencilBlock;
     __shared__ float tile[2][LN];
+    const int t = (int)(blockIdx.x % kStencilXCD) * tl.per_xcd + (int)(blockIdx.x / kStencilXCD);
+    if (t >= tl.nbx * tl.nby * tl.nbz) return;  // the whole workgroup leaves before any barrier
+    const int tbx = t % tl.nbx, tby = (t / tl.nbx) % tl.nby, tbz = t / (tl.nbx * tl.nby);
     const int lane = threadIdx.x & 63, wy = threadIdx.x >> 6;
-    const int x0 = g.nbuff + 64 * blockIdx.x, y0 = g.nbuff + kStencilTY * blockIdx.y;
-    const int z0 = g.nbuff + kStencilKZ * blockIdx.z;
+    const int x0 = g.nbuff + 64 * tbx, y0 = g.nbuff + kStencilTY * tby;
+    const int z0 = g.nbuff + kStencilKZ * tbz;
     const int z1 = min(z0 + kStencilKZ, g.nbuff + g.in3);  // planes z0..z1-1; z0-1..z1 are read
     const long long s12 = (long long)g.n1 * g.n2;
     const int x = x0 + lane;
@@ -242,7 +256,7 @@ __global__ void __launch_bounds__(kStencilBlock) k_peak_bits(const float *__rest
         if (ka) ka = mask[x + (long long)ya * g.n1 + (long long)z * s12] != 1;
         if (kb) kb = mask[x + (long long)yb * g.n1 + (long long)z * s12] != 1;
         const unsigned long long wa = __ballot(ka), wb = __ballot(kb);
-        const long long base = (long long)(z - g.nbuff) * word_row + blockIdx.x;
+        const long long base = (long long)(z - g.nbuff) * word_row + tbx;
         if (lane == 0 && ya < g.nbuff + g.in2) bits[base + (long long)(ya - g.nbuff) * g.words] = wa;
         if (lane == 0 && yb < g.nbuff + g.in2) bits[base + (long long)(yb - g.nbuff) * g.words] = wb;
         pa = qa; qa = ma; ca = na_c;
@@ -375,10 +389,15 @@ static int run_stencil_with(pp_ctx *ctx, const float *d_field, int8_t *d_mask, c
         PP_HIP(hipGetLastError());
         return PP_OK;
     }
-    dim3 grid(g.words, (g.in2 + kStencilTY - 1) / kStencilTY, (g.in3 + kStencilKZ - 1) / kStencilKZ);
-    PP_ARG(grid.y <= 65535 && grid.z <= 65535, "inner cube too large for the stencil grid");
-    hipLaunchKernelGGL(k_peak_bits, grid, dim3(kStencilBlock), 0, ctx->stream, d_field, (const int8_t *)d_mask, g,
-                       fcrit, s.bits);
+    StencilTiles tl;
+    tl.nbx = g.words;
+    tl.nby = (g.in2 + kStencilTY - 1) / kStencilTY;
+    tl.nbz = (g.in3 + kStencilKZ - 1) / kStencilKZ;
+    const long long ntiles = (long long)tl.nbx * tl.nby * tl.nbz;
+    PP_ARG(ntiles < (1ll << 30), "inner cube too large for the stencil grid");
+    tl.per_xcd = (int)((ntiles + kStencilXCD - 1) / kStencilXCD);
+    hipLaunchKernelGGL(k_peak_bits, dim3((unsigned)tl.per_xcd * kStencilXCD), dim3(kStencilBlock), 0, ctx->stream,
+                       d_field, (const int8_t *)d_mask, g, tl, fcrit, s.bits);
     PP_HIP(hipGetLastError());
     hipcub::TransformInputIterator<int, Popc, const uint64_t *> it(s.bits, Popc());
     size_t tmp = s.tmp_bytes;

@@ -462,10 +462,34 @@ int pp_split_tile_fields(pp_ctx *ctx, const pp_coarse *c, int it, int jt, int kt
     // 1. residual -> padded buffer R; base spectrum in place
     FftPlans *Fi = nullptr;
     PP_CHECK(fft_plans(ctx, n2, 1, &Fi));
+    // the two padded (2n)^3 buffers (the largest of the tile) are stream
+    // ordered and returned when the tile's fields are written, so a context
+    // does not keep them between tiles
     const size_t pad_floats = (size_t)(n2 + 2) * n2 * n2;
-    PP_CHECK(ctx->d_iso[0].grow(pad_floats * 4));
-    PP_CHECK(ctx->d_iso[1].grow(pad_floats * 4));
-    float *R = ctx->d_iso[0].as<float>(), *S = ctx->d_iso[1].as<float>();
+    float *R = nullptr, *S = nullptr;
+    PP_HIP(hipMallocAsync((void **)&R, pad_floats * 4, ctx->stream));
+    hipError_t ea = hipMallocAsync((void **)&S, pad_floats * 4, ctx->stream);
+    if (ea != hipSuccess) {
+        (void)hipFreeAsync(R, ctx->stream);
+        set_error("hipMallocAsync of %zu B: %s", pad_floats * 4, hipGetErrorString(ea));
+        return PP_ERR_HIP;
+    }
+    void *Wk = nullptr;
+    struct Release {
+        hipStream_t s;
+        float *a, *b;
+        void **w;
+        ~Release()
+        {
+            (void)hipFreeAsync(a, s);
+            (void)hipFreeAsync(b, s);
+            if (*w) (void)hipFreeAsync(*w, s);
+        }
+    } release{ctx->stream, R, S, &Wk};
+    if (Fi->work_need > 0) {
+        PP_HIP(hipMallocAsync(&Wk, Fi->work_need, ctx->stream));
+        PP_FFT(rocfft_execution_info_set_work_buffer(Fi->info, Wk, Fi->work_need));
+    }
     TileGeo T;
     T.n = n;
     T.N = c->N;

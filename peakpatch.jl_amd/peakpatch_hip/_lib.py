@@ -132,6 +132,7 @@ SIGNATURES = {
     "pp_last_shell_stages": (_i, [_vp, C.POINTER(_d)]),
     "pp_fill_noise_threefry": (_i, [_vp, _vp, _i64, C.c_uint64]),
     "pp_make_tile_fields": (_i, [_vp, _i, _d, C.c_uint64, C.POINTER(PkTable), C.POINTER(TileFields)]),
+    "pp_ctx_release_scratch": (_i, [_vp]),
     "pp_coarse_create": (_i, [_vp, C.POINTER(SplitGeom), C.POINTER(PkTable), C.POINTER(C.c_void_p)]),
     "pp_coarse_download": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pp_coarse_destroy": (_i, [_vp]),

@@ -141,6 +141,11 @@ class Context:
         L.check(self.lib.pp_ctx_signal_stream(self.h, s))
         return rc
 
+    def release_scratch(self):
+        """Return the context's pooled working memory to the device (tables
+        kept); waits for the context's streams."""
+        L.check(self.lib.pp_ctx_release_scratch(self.h))
+
     def fill_noise(self, n: int, seed: int, out=None):
         """Threefry white noise of an n^3 tile as a CUDA float32 tensor
         (RandomField.jl:46-81), generated on the context's stream and

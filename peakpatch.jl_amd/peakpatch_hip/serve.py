@@ -43,15 +43,21 @@ from .pipeline import TileConfig
 SERVE_TEMPLATE = TileConfig("c5", 256, 250.0, 20, 1.0, 15.0, ilpt=2, ioutshear=1, ievol=1, z_max=4.6,
                             cen=(0.0, 0.0, 0.0))
 SERVE_NMESH = (128, 256, 384, 512, 768)
+# run_multitile_split runs behind the stream: one global field per nmesh,
+# SPLIT_NTILE^3 tiles each, coarse grid COARSE_FACTOR x SPLIT_NTILE
+SPLIT_NTILE = 4
+COARSE_FACTOR = 5
+SPLIT_SEED = 42
 
 
 @dataclass
 class TileRequest:
     tile_id: int
     nmesh: int
-    center: tuple = (0.0, 0.0, 0.0)
-    seed: int = 42
-    arrival: float = 0.0            # perf_counter() at submission (set by submit)
+    center: tuple = (0.0, 0.0, 0.0)  # periodic tiles: the tile centre
+    seed: int = 42                   # periodic tiles: the tile's own field
+    arrival: float = 0.0             # perf_counter() at submission (set by submit)
+    tile: tuple | None = None        # split tiles: (it, jt, kt), 1-based, in the nmesh's global grid
 
 
 @dataclass
@@ -89,10 +95,11 @@ def footprint_bytes(n: int) -> int:
 
 
 def poisson_stream(ntiles: int, rate: float, seed: int = 0, nmesh=SERVE_NMESH, weights=None,
-                   extent: float = 1500.0):
+                   extent: float = 1500.0, ntile: int = SPLIT_NTILE):
     """(arrival offset s, TileRequest) pairs: exponential gaps at `rate`
-    tiles/s, nmesh drawn from `nmesh`, tile centres uniform in a light-cone
-    cube of side `extent` Mpc/h beside the observer."""
+    tiles/s, nmesh drawn from `nmesh`, each request a uniformly drawn tile
+    (it, jt, kt) of that nmesh's split run (and, for periodic tiles, a centre
+    uniform in a light-cone cube of side `extent` Mpc/h)."""
     rng = np.random.default_rng(seed)
     w = None if weights is None else np.asarray(weights, float) / np.sum(weights)
     t = 0.0
@@ -101,7 +108,8 @@ def poisson_stream(ntiles: int, rate: float, seed: int = 0, nmesh=SERVE_NMESH, w
         t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
         n = int(rng.choice(nmesh, p=w))
         c = tuple(float(v) for v in rng.uniform(0.0, extent, 3))
-        out.append((t, TileRequest(q, n, c, seed=1000 + q)))
+        tl = tuple(int(v) for v in rng.integers(1, ntile + 1, 3))
+        out.append((t, TileRequest(q, n, c, seed=1000 + q, tile=tl)))
     return out
 
 
@@ -127,13 +135,17 @@ class TileServer:
         from . import _lib as L
         self._L = L
         self._lib = L.load()
-        self._footprint = footprint
         self._runner = None
+        self._own_runner = False
         native_ctx = run_tile is None
         if native_ctx:
+            self._own_runner = runner is None
             self._runner = runner if runner is not None else GpuTileRunner(device)
             run_tile = self._runner.run
-        elif slot_states is None:
+            if footprint is footprint_bytes:
+                footprint = self._runner.footprint
+        self._footprint = footprint
+        if not native_ctx and slot_states is None:
             slot_states = [None] * slots
         self._run = run_tile
         self._lock = threading.Lock()
@@ -217,6 +229,8 @@ class TileServer:
         if self._runner is not None:
             for s in self._states:
                 self._runner.release_slot(s)
+            if self._own_runner:
+                self._runner.close()
 
     def __enter__(self):
         return self
@@ -299,11 +313,29 @@ class GpuTileRunner:
     """The per-tile GPU pipeline on one slot: synthesis -> filter bank +
     stencil -> per-peak bookkeeping -> shell analysis -> record packing, all
     on the slot context's stream; the peak list and the per-peak batch stay
-    on the device."""
+    on the device.
+
+    split=True (c5 as run_multitile_split serves it): a request names a tile
+    (it, jt, kt) of its nmesh's global field; the fields come from that
+    field's coarse grid (created on first use, then resident on the device
+    for the runner's life and shared by every slot) and the tile's residual
+    (pp_split_tile_fields), nbuff = the split geometry's.  split=False: each
+    request is a periodic tile of its own seed (pp_make_tile_fields)."""
 
     def __init__(self, device: int = 0, template: TileConfig = SERVE_TEMPLATE, table=None, tunables=None,
-                 round_f32: bool = True):
+                 round_f32: bool = True, split: bool = False, ntile: int = SPLIT_NTILE,
+                 coarse_factor: int = COARSE_FACTOR, split_seed: int = SPLIT_SEED, trim_n: int = 512):
+        import threading as _th
+
         from . import collapse
+        self.split = split
+        self.ntile, self.coarse_factor, self.split_seed = ntile, coarse_factor, split_seed
+        # after a tile of nmesh >= trim_n the slot returns its pooled memory
+        # (the context's scratch and torch's cached blocks), so an idle slot
+        # does not hold a large tile's working set against the other slots
+        self.trim_n = trim_n
+        self._coarse = {}
+        self._coarse_lock = _th.Lock()
         self.device = device
         # run_multitile_split stores per-peak ZZon and fcrit as Float32
         # (MultiResolution.jl:963-967)
@@ -331,6 +363,31 @@ class GpuTileRunner:
         if state is not None:
             state["ctx"].close()
 
+    def split_geometry(self, nmesh: int):
+        from . import split as S
+        return S.geometry(nmesh, self.template.cap + 1, self.ntile, self.template.alatt,
+                          coarse_factor=self.coarse_factor, seed=self.split_seed)
+
+    def coarse(self, ctx, nmesh: int):
+        """The resident coarse field of nmesh's split run (made on first use)."""
+        from . import split as S
+        with self._coarse_lock:
+            c = self._coarse.get(nmesh)
+            if c is None:
+                c = S.CoarseField(ctx, self.split_geometry(nmesh))
+                self._coarse[nmesh] = c
+            return c
+
+    def footprint(self, nmesh: int) -> int:
+        from . import split as S
+        return S.footprint_bytes(nmesh) if self.split else footprint_bytes(nmesh)
+
+    def close(self):
+        with self._coarse_lock:
+            for c in self._coarse.values():
+                c.close()
+            self._coarse.clear()
+
     def run(self, state, req: TileRequest) -> TileResult:
         import torch
         from . import catalog, synth
@@ -338,6 +395,11 @@ class GpuTileRunner:
         from .pipeline import TileDriver
         ctx, stream = state["ctx"], state["stream"]
         cfg = tile_config(req, self.template)
+        if self.split:
+            from . import split as S
+            g = self.split_geometry(req.nmesh)
+            tix = req.tile if req.tile is not None else (1, 1, 1)
+            cfg = replace(cfg, nbuff=g.nbuff, tile_center=g.tile_center(*tix), seed=g.seed)
         drv = TileDriver(cfg)
         st = {}
         t = [time.perf_counter()]
@@ -349,7 +411,12 @@ class GpuTileRunner:
             t[0] = now
 
         with torch.cuda.stream(stream):
-            tile = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt, want_lapd=cfg.ioutshear >= 1)
+            if self.split:
+                tile = S.make_split_tile(ctx, self.coarse(ctx, req.nmesh), *tix, ilpt=cfg.ilpt,
+                                         want_lapd=cfg.ioutshear >= 1)
+            else:
+                tile = synth.make_tile(ctx, cfg.n, cfg.boxsize, cfg.seed, ilpt=cfg.ilpt,
+                                       want_lapd=cfg.ioutshear >= 1)
             lap("synthesis")
             peaks = drv.find_peaks(ctx, tile.delta, device_out=True)
             lap("peak_find")
@@ -368,4 +435,8 @@ class GpuTileRunner:
             lap("records")
         out.RTHLmax = rmax
         out.nhalo = len(out.records)
+        if self.trim_n and cfg.n >= self.trim_n:
+            del tile, peaks, bdev, mask, fields, res, rec
+            ctx.release_scratch()
+            torch.cuda.empty_cache()
         return out

@@ -105,3 +105,23 @@ def test_bad_tile_index_is_refused(ctx):
     with pytest.raises(PPError):
         split.make_split_tile(ctx, cf, 3, 1, 1)
     cf.close()
+
+
+def test_release_scratch_keeps_results(ctx):
+    """pp_ctx_release_scratch returns the pooled memory; the next calls grow
+    the pools again and give the same fields and peaks."""
+    import torch
+    from peakpatch_hip import split
+    from peakpatch_hip.pipeline import CONFIGS, TileDriver, scaled
+    g = split.SplitGeom(*GEOMS["cf5"], seed=3)
+    cf = split.CoarseField(ctx, g)
+    cfg = scaled(CONFIGS["c2"], g.nmesh)
+    a = split.make_split_tile(ctx, cf, 1, 2, 1)
+    pa = TileDriver(cfg).find_peaks(ctx, a.delta)
+    ctx.release_scratch()
+    b = split.make_split_tile(ctx, cf, 1, 2, 1)
+    pb = TileDriver(cfg).find_peaks(ctx, b.delta)
+    torch.cuda.synchronize()
+    assert torch.equal(a.delta, b.delta) and torch.equal(a.eta2[1], b.eta2[1])
+    assert len(pa) > 0 and np.array_equal(pa.i, pb.i) and np.array_equal(pa.delta_s, pb.delta_s)
+    cf.close()
