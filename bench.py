@@ -59,6 +59,7 @@ def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
     walls, kern, stages = [], [], []
     res = None
     for _ in range(steps):
+        res = None  # the previous step's result arrays go back to the allocator before this step
         mask.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -152,26 +153,41 @@ def serve_stream(args, ct, tuned):
     resolution, each generated from its nmesh's resident coarse field plus its
     own isolated residual, peak-found, bookkept, shell-analysed and packed on
     the GPU, on the native server's slot contexts with admission by device
-    memory.  Two runs of
-    args.serve_tiles tiles: all submitted at once (the drain rate is the
-    saturation capacity), then Poisson arrivals at args.serve_load times that
-    capacity (or at args.serve_rate); latency = completion - arrival,
-    queueing included."""
+    memory.  Two runs:
+    args.serve_burst tiles submitted at once (the drain rate is the saturation
+    capacity), then args.serve_tiles Poisson arrivals at args.serve_load
+    times that capacity (or at args.serve_rate); latency = completion -
+    arrival, queueing included."""
     from peakpatch_hip import serve
     runner = serve.GpuTileRunner(table=ct, tunables=tuned, split=True)
     n = args.serve_tiles
+    done = [0]
+
+    def progress(what):
+        def cb(r):
+            done[0] += 1
+            if done[0] % 25 == 0 or r.error:
+                log(f"[serve] {what}: {done[0]} tiles done (last nmesh {r.nmesh}, "
+                    f"{'error: ' + r.error if r.error else f'{r.latency * 1e3:.0f} ms'})")
+        return cb
     with serve.TileServer(slots=args.serve_slots, runner=runner, hbm_fraction=0.8) as srv:
         # warm the slots on every nmesh (the resident coarse fields, transform
         # plans, buffer pools), untimed
         warm = [serve.TileRequest(-1, m, tile=(1, 1, 1)) for m in serve.SERVE_NMESH for _ in range(args.serve_slots)]
-        bad = [r.error for r in serve.replay(srv, [(0.0, r) for r in warm]) if r.error]
+        t0 = time.perf_counter()
+        bad = [r.error for r in serve.replay(srv, [(0.0, r) for r in warm], on_done=progress("warm-up")) if r.error]
         if bad:
             raise RuntimeError(f"serving warm-up failed: {bad[0]}")
-        burst = serve.poisson_stream(n, 0.0, seed=5)
-        cap = serve.summarize(serve.replay(srv, burst))
+        log(f"[serve] warm-up ({len(warm)} tiles, coarse fields of {len(serve.SERVE_NMESH)} runs) "
+            f"{time.perf_counter() - t0:.1f} s")
+        done[0] = 0
+        burst = serve.poisson_stream(args.serve_burst, 0.0, seed=5)
+        cap = serve.summarize(serve.replay(srv, burst, on_done=progress("capacity run")))
+        log(f"[serve] capacity {cap['tiles_per_s']:.2f} tiles/s over {cap['wall_s']:.1f} s")
+        done[0] = 0
         rate = args.serve_rate if args.serve_rate > 0 else args.serve_load * cap["tiles_per_s"]
         stream = serve.poisson_stream(n, rate, seed=6)
-        res = serve.replay(srv, stream)
+        res = serve.replay(srv, stream, on_done=progress(f"Poisson {rate:.2f}/s"))
         st = srv.stats()
     geoms = {m: runner.split_geometry(m) for m in serve.SERVE_NMESH}
     runner.close()
@@ -213,8 +229,10 @@ def main():
                     help="before the timed steps, run one elementwise kernel over a GIB-sized float32 buffer "
                          "(known traffic, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--serve-tiles", type=int, default=200,
-                    help="tiles in each serving run (nmesh in {128..768}); 0 skips serving")
-    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="tiles in the serving latency run (nmesh in {128..768}); 0 skips serving")
+    ap.add_argument("--serve-burst", type=int, default=60,
+                    help="tiles submitted at once to measure the serving capacity (drain rate)")
+    ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson arrival rate of the latency run, as a fraction of the measured capacity")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate in tiles/s (overrides --serve-load when > 0)")

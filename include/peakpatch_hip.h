@@ -85,9 +85,8 @@ int pp_ctx_signal_stream(pp_ctx *ctx, void *stream);
 /* Free / total device memory as seen by this context (for admission). */
 int pp_ctx_mem_info(pp_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 /* Returns the context's pooled working memory (field, spectrum, peak and
- * profile scratch, transform plans and their work areas) to the device,
- * keeping the shell and collapse tables; the next call grows the pools
- * again.  Waits for the context's streams.  No counterpart in the
+ * profile scratch) to the device, keeping the shell, gather and collapse
+ * tables and the transform plans; the next call grows the pools again.  Waits for the context's streams.  No counterpart in the
  * reference: a serving loop trims an idle slot after a large tile so the
  * memory is free for tiles on other slots. */
 int pp_ctx_release_scratch(pp_ctx *ctx);

@@ -287,14 +287,11 @@ int pp_ctx_release_scratch(pp_ctx *ctx)
     PP_CHECK(ensure_device(ctx));
     PP_HIP(hipStreamSynchronize(ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->side));
-    for (FftPlans &F : ctx->fft_cache) F.release();
-    ctx->fftp = nullptr;
     DevBuf *bufs[] = {&ctx->d_field_a, &ctx->d_field_b, &ctx->d_k0, &ctx->d_k1, &ctx->d_k2, &ctx->d_mask,
                       &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan, &ctx->d_cub, &ctx->d_pk, &ctx->d_res,
                       &ctx->d_prof, &ctx->d_queue};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : ctx->d_in) b.release();
-    persist_release(ctx);
     return PP_OK;
 }
 

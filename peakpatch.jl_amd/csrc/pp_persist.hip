@@ -75,7 +75,7 @@ constexpr int NXCD = 8;
 // With eta2, (eta z, eta2 x, eta2 y, eta2 z) go to a second float4 array
 // (q4) for the small-peak gather and the finish pass.  The block stages
 // delta(z, x, y) for 32 x 32 (x, z) in LDS (the transposed read).
-constexpr int kPackTile = 32;  // x and z extent of k_pack4's transpose tile (the fields' row runs are T floats)
+constexpr int kPackTile = 64;  // x and z extent of k_pack4's transpose tile (the fields' row runs are T floats)
 template <int T>
 __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, const float *__restrict__ ex,
                                                const float *__restrict__ ey, const float *__restrict__ ez,
@@ -322,11 +322,20 @@ __device__ __forceinline__ double wave_scan(double x)
 // lane += lane + d while inside the run), and the run's first lane adds the
 // sum to that shell's LDS bin.  Records carry S, S2 and lapd so the lane
 // post can finish these peaks itself.
+// A lane plays two roles per batch.  Loading, it reads the cell at its
+// position in the batch's row-major order (lcell, lq), so a load instruction
+// covers whole x-runs of the peaks' rows; its contributions go through LDS to
+// the lane at the cell's position in (peak, shell) order, which sums runs of
+// equal (peak, shell) with the tree (cell, q, s, run, rend; src = the lane
+// that loaded this position's cell).
 struct CellSlot {
     int32_t cell;   // (di+128) | (dj+128) << 8 | (dk+128) << 16; -1 for an empty lane
     int16_t q, s;   // peak slot in the block, shell (1-based)
     int32_t run;    // length of the (q, s) run starting here, 0 if not its first lane
     int32_t rend;   // lane one past the end of this lane's run
+    int32_t lcell;  // the cell this lane loads (row-major position); -1 for none
+    int16_t lq;     // its peak slot
+    int16_t src;    // the lane that loads this (peak, shell)-ordered position's cell
 };
 
 struct SmallSeg {
@@ -362,6 +371,8 @@ __device__ __forceinline__ constexpr int sf_slot(int f) { return f < P_NB ? f : 
 __global__ void __launch_bounds__(64) k_small(SmallArgs A)
 {
     __shared__ float bins[NSF][64];  // (peak q, shell s) at q m + s - 1
+    __shared__ float stage[NSF][64]; // a batch's contributions, by loading lane
+    __shared__ int4 spk[64];         // centre and flags of the block's peaks
     __shared__ double sdF[64], sfb[64];
     const int lane = threadIdx.x;
     const GatherFields &F = A.F;
@@ -376,19 +387,23 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
         const int m = S.m, P = S.P;
         const int64_t pbase = S.lo + (pt.blk_lo + (j - pt.cum)) * P;
         for (int f = 0; f < NSF; f++) bins[f][lane] = 0.0f;
+        if (lane < P && pbase + lane < S.hi) {
+            const PeakParam pk = A.pps[pbase + lane];
+            spk[lane] = make_int4(pk.ci, pk.cj, pk.ck, pk.flags);
+        }
         __syncthreads();
         for (int b = 0; b < S.nbat; b++) {
             const CellSlot t = A.tab[S.toff + b * 64 + lane];
-            const int64_t idx = pbase + t.q;
-            const bool live = t.cell >= 0 && idx < S.hi;
+            const bool live = t.cell >= 0 && pbase + t.q < S.hi;
             float acc[NSF];
 #pragma unroll
             for (int f = 0; f < NSF; f++) acc[f] = 0.0f;
-            if (live) {
-                const PeakParam pk = A.pps[idx];
-                const int dx = (t.cell & 255) - 128, dy = ((t.cell >> 8) & 255) - 128, dz = ((t.cell >> 16) & 255) - 128;
-                const int x = pk.ci + dx, y = pk.cj + dy, z = pk.ck + dz;
-                if ((pk.flags & 1) || (x >= 0 && x < F.n && y >= 0 && y < F.n && z >= 0 && z < F.n)) {
+            if (t.lcell >= 0 && pbase + t.lq < S.hi) {
+                const int4 pk = spk[t.lq];
+                const int dx = (t.lcell & 255) - 128, dy = ((t.lcell >> 8) & 255) - 128,
+                          dz = ((t.lcell >> 16) & 255) - 128;
+                const int x = pk.x + dx, y = pk.y + dy, z = pk.z + dz;
+                if ((pk.w & 1) || (x >= 0 && x < F.n && y >= 0 && y < F.n && z >= 0 && z < F.n)) {
                     const long a = x + y * n1 + z * n12;
                     const float4 q = F.p4[a];
                     float4 q2;
@@ -412,6 +427,12 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
                     if (F.lap) acc[P_NB + 6] = F.lap[a];
                 }
             }
+            // row-major loads -> (peak, shell) order
+#pragma unroll
+            for (int f = 0; f < NSF; f++) stage[f][lane] = acc[f];
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < NSF; f++) acc[f] = stage[f][t.src];
             for (int k = 0; k < S.steps; k++) {
                 const int d = 1 << k;
                 const bool in = lane + d < t.rend;
@@ -1040,11 +1061,17 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
                     c.s = cells[v % nc].second;
                     c.run = 0;
                     c.rend = 0;
+                    c.src = (int16_t)bt.size();  // the loading lane: its row-major position
                     bt.push_back(c);
                 }
+                const std::vector<CellSlot> rowmajor = bt;
                 std::stable_sort(bt.begin(), bt.end(), [](const CellSlot &x, const CellSlot &y) {
                     return x.q != y.q ? x.q < y.q : x.s < y.s;
                 });
+                for (size_t u = 0; u < bt.size(); u++) {
+                    bt[u].lcell = rowmajor[u].cell;
+                    bt[u].lq = rowmajor[u].q;
+                }
                 for (size_t q0 = 0; q0 < bt.size();) {
                     size_t r = q0;
                     while (r < bt.size() && bt[r].q == bt[q0].q && bt[r].s == bt[q0].s) r++;
@@ -1053,7 +1080,8 @@ static int ensure_tables(pp_ctx *ctx, const std::vector<int> &ms, int cs)
                     while ((1 << t.steps) < (int)(r - q0)) t.steps++;
                     q0 = r;
                 }
-                while (bt.size() < 64) bt.push_back(CellSlot{-1, 0, 1, 0, (int32_t)bt.size() + 1});
+                while (bt.size() < 64)
+                    bt.push_back(CellSlot{-1, 0, 1, 0, (int32_t)bt.size() + 1, -1, 0, (int16_t)bt.size()});
                 st.scells.insert(st.scells.end(), bt.begin(), bt.end());
             }
             st.stab[m] = t;

@@ -279,16 +279,20 @@ class TileServer:
             fut.set_result(res)
 
 
-def replay(server: TileServer, stream, time_scale: float = 1.0):
+def replay(server: TileServer, stream, time_scale: float = 1.0, on_done=None):
     """Submit (offset, request) pairs at their offsets; return the results in
-    request order once all have finished."""
+    request order once all have finished.  on_done(result) is called as each
+    tile finishes (from the completion thread)."""
     t0 = time.perf_counter()
     futs = []
     for off, req in stream:
         dt = t0 + off * time_scale - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
-        futs.append(server.submit(req))
+        f = server.submit(req)
+        if on_done is not None:
+            f.add_done_callback(lambda fu: on_done(fu.result()))
+        futs.append(f)
     return [f.result() for f in futs]
 
 
