@@ -26,7 +26,7 @@ namespace pp {
 // noise_k * sqrt(P(k) dk^3 n^3) / n^3, k = 0 zeroed (RandomField.jl:144-167)
 __global__ void __launch_bounds__(256) k_grf_amp(float2 *__restrict__ a, KGrid g, long long total, double dk3,
                                                  double n3, double inv_n3, const double *__restrict__ lk,
-                                                 const double *__restrict__ lp, int np)
+                                                 const double *__restrict__ lp, int np, double lk0, double inv_dlk)
 {
 #pragma clang fp contract(off)
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(256) k_grf_amp(float2 *__restrict__ a, KGrid g
             a[e] = make_float2(0.f, 0.f);
             continue;
         }
-        const double amp = sqrt(pk_eval(lk, lp, np, sqrt(k2)) * dk3 * n3);
+        const double amp = sqrt(pk_eval(lk, lp, np, sqrt(k2), lk0, inv_dlk) * dk3 * n3);
         a[e] = cmul(a[e], amp * inv_n3, 0.0);
     }
 }
@@ -165,7 +165,8 @@ extern "C" int pp_make_tile_fields(pp_ctx *ctx, int n, double boxsize, uint64_t 
     PP_CHECK(launch_noise(ctx, f->delta, (int64_t)N, seed));
     PP_CHECK(r2c(f->delta, dk0));
     hipLaunchKernelGGL(k_grf_amp, dim3(grid), dim3(256), 0, ctx->stream, dk0, g, total, dk * dk * dk,
-                       (double)n * n * n, inv_n3, (const double *)d_lk, (const double *)d_lp, pk->npk);
+                       (double)n * n * n, inv_n3, (const double *)d_lk, (const double *)d_lp, pk->npk, lk[0],
+                       pk_uniform_inv_spacing(lk.data(), pk->npk));
     PP_HIP(hipGetLastError());
     PP_CHECK(c2r(dk0, f->delta));
     // 2. delta_k = rfft(delta), kept in dk0

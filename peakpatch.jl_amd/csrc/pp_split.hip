@@ -16,7 +16,9 @@
 //   fixed tree;
 // * the residual is written straight into the zero-padded real buffer of an
 //   in-place (2n)^3 rocFFT transform (rows padded to 2 (n+1) floats), two
-//   cells per lane;
+//   cells per lane.  (Transforming only the planes that hold data, with the
+//   z axis as a separate strided 1-D pass, measured slower: rocFFT's strided
+//   1-D z pass costs more than its whole 3-D plan saves.);
 // * each output's spectrum is one pass over the base spectrum that forms the
 //   Float64 transfer factor, rounds the product into ComplexF32 once and,
 //   on the kx = 0 and kx = Nyquist planes, averages the mode with its
@@ -39,6 +41,7 @@ struct pp_coarse {
     pp_split_geom g{};
     int N = 0, nsub = 0, block = 0, npk = 0;
     double boxsize_full = 0;
+    double lk0 = 0, inv_dlk = 0;  // the P(k) table's first log k and 1 / spacing when even
     pp::DevBuf noise, delta, psi, pk;  // M^3 floats; 3 M^3 floats; log k, log P
 };
 
@@ -93,6 +96,7 @@ struct Xfer {
     double dk3, n3;  // dk^3 and n^3 of the amplitude sqrt((P(k) dk^3) n^3)
     const double *lk, *lp;
     int np;
+    double lk0, inv_dlk;  // evenly spaced table: first log k and 1 / spacing (0: bisection)
 };
 
 // the mode at (ix, iy, iz) times its transfer factor, rounded to ComplexF32
@@ -108,11 +112,11 @@ __device__ __forceinline__ float2 xfer_mode(const Xfer &X, const float2 *__restr
     const float2 v = src[ix + (long long)g.nk * (iy + (long long)g.n * iz)];
     switch (X.mode) {
     case XF_AMP: {
-        const double amp = sqrt(pk_eval(X.lk, X.lp, X.np, sqrt(k2)) * X.dk3 * X.n3);
+        const double amp = sqrt(pk_eval(X.lk, X.lp, X.np, sqrt(k2), X.lk0, X.inv_dlk) * X.dk3 * X.n3);
         return cmul(v, amp, 0.0);
     }
     case XF_AMP_IK: {
-        const double amp = sqrt(pk_eval(X.lk, X.lp, X.np, sqrt(k2)) * X.dk3 * X.n3);
+        const double amp = sqrt(pk_eval(X.lk, X.lp, X.np, sqrt(k2), X.lk0, X.inv_dlk) * X.dk3 * X.n3);
         return cmul(v, 0.0, amp * (k[X.a] / k2));
     }
     case XF_PHI:
@@ -168,12 +172,13 @@ struct TileGeo {
 // two cells of the coarse one (M >= 2)
 __device__ __forceinline__ int wrap1(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
 
-// residual into the zero-padded (2n)^3 real buffer of pitch 2 (n + 1)
+// residual into the zero-padded (2n)^3 real buffer of pitch 2 (n + 1), every
+// float of every row written
 __global__ void __launch_bounds__(256) k_residual_pad(float *__restrict__ R, TileGeo T,
                                                       const float *__restrict__ coarse)
 {
-    const int n2 = 2 * T.n, pitch = n2 + 2, half = n2 / 2;
-    const long long total = (long long)half * n2 * n2;  // two cells per item
+    const int n2 = 2 * T.n, pitch = n2 + 2, half = pitch / 2;
+    const long long total = (long long)half * n2 * n2;  // two floats per item, padding floats included
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
         const int x = 2 * (int)(e % half);
@@ -228,13 +233,23 @@ __device__ __forceinline__ void cr_coord(int g, int b, int &i0, float &t)
 }
 
 // out = (padded inverse transform at the tile cell) * inv + Catmull-Rom of
-// the coarse field
+// the coarse field.  LDS: the whole coarse field is first copied to LDS (its
+// M^3 floats fit when M^3 <= kCoarseLds), so the 64 taps per cell are LDS reads.
+constexpr int kCoarseLds = 16384;
+
+template <bool LDS>
 __global__ void __launch_bounds__(256) k_extract_add(const float *__restrict__ S, float inv, TileGeo T,
-                                                     const float *__restrict__ coarse, float *__restrict__ out)
+                                                     const float *__restrict__ coarse_g, float *__restrict__ out)
 {
 #pragma clang fp contract(off)
     const int n = T.n, n2 = 2 * n, pitch = n2 + 2, M = T.M;
     const long long total = (long long)n * n * n;
+    extern __shared__ float coarse_s[];
+    if (LDS) {
+        for (int q = threadIdx.x; q < M * M * M; q += blockDim.x) coarse_s[q] = coarse_g[q];
+        __syncthreads();
+    }
+    const float *coarse = LDS ? coarse_s : coarse_g;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
         const int x = (int)(e % n);
@@ -318,6 +333,8 @@ static Xfer make_xfer(int n, double dk, int mode, int a, int b, int nyq_zero, co
         X.lk = c->pk.as<double>();
         X.lp = X.lk + c->npk;
         X.np = c->npk;
+        X.lk0 = c->lk0;
+        X.inv_dlk = c->inv_dlk;
     }
     return X;
 }
@@ -372,6 +389,8 @@ int pp_coarse_create(pp_ctx *ctx, const pp_split_geom *geom, const pp_pk_table *
         lt[q] = log(pk->k[q]);
         lt[pk->npk + q] = log(pk->P[q]);
     }
+    c->lk0 = lt[0];
+    c->inv_dlk = pk_uniform_inv_spacing(lt.data(), pk->npk);
     hipError_t e = hipMemcpyAsync(c->pk.ptr, lt.data(), lt.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream);
     if (e != hipSuccess) {
@@ -500,7 +519,7 @@ int pp_split_tile_fields(pp_ctx *ctx, const pp_coarse *c, int it, int jt, int kt
     T.k0 = (kt - 1) * c->nsub - G.nbuff;
     T.inv_b3 = (float)(1.0 / sqrt(pow((double)c->block, 3)));
     T.seed = G.seed;
-    hipLaunchKernelGGL(k_residual_pad, dim3(grid_for(ctx, (long long)n * n2 * n2)), dim3(256), 0, ctx->stream, R, T,
+    hipLaunchKernelGGL(k_residual_pad, dim3(grid_for(ctx, (long long)(n + 1) * n2 * n2)), dim3(256), 0, ctx->stream, R, T,
                        (const float *)c->noise.ptr);
     PP_HIP(hipGetLastError());
     {
@@ -518,8 +537,13 @@ int pp_split_tile_fields(pp_ctx *ctx, const pp_coarse *c, int it, int jt, int kt
         void *io[1] = {S};
         PP_FFT(rocfft_execute(Fi->c2r, io, nullptr, Fi->info));
         const float *cf = q == 0 ? c->delta.as<float>() : c->psi.as<float>() + (long long)(q - 1) * G.M * G.M * G.M;
-        hipLaunchKernelGGL(k_extract_add, dim3(grid_for(ctx, n3)), dim3(256), 0, ctx->stream, (const float *)S, inv_i,
-                           T, cf, outs[q]);
+        const long long M3 = (long long)G.M * G.M * G.M;
+        if (M3 <= kCoarseLds)
+            hipLaunchKernelGGL(k_extract_add<true>, dim3(grid_for(ctx, n3)), dim3(256), M3 * sizeof(float),
+                               ctx->stream, (const float *)S, inv_i, T, cf, outs[q]);
+        else
+            hipLaunchKernelGGL(k_extract_add<false>, dim3(grid_for(ctx, n3)), dim3(256), 0, ctx->stream,
+                               (const float *)S, inv_i, T, cf, outs[q]);
         PP_HIP(hipGetLastError());
     }
     // 3. the tile-periodic 2LPT and k^2 fields from the combined delta

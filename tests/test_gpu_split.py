@@ -27,9 +27,11 @@ def _pk():
 
 
 GEOMS = {
-    # nmesh, nbuff, ntile, M, alatt: block 8 (coarse_factor 5), and block 5 (odd) with a middle tile
+    # nmesh, nbuff, ntile, M, alatt: block 8 (coarse_factor 5); block 5 (odd) with a middle tile;
+    # a coarse grid too large for LDS (32^3 floats: the interpolation reads it from global memory)
     "cf5": (48, 8, 2, 10, 2.0),
     "odd": (40, 5, 3, 20, 1.5),
+    "big": (32, 0, 2, 32, 2.0),
 }
 
 
@@ -59,7 +61,7 @@ def test_coarse_fields_match_the_oracle(ctx, gname):
 
 
 @pytest.mark.parametrize("gname,tile", [("cf5", (1, 1, 1)), ("cf5", (2, 1, 2)), ("odd", (2, 3, 1)),
-                                        ("odd", (3, 2, 2))])
+                                        ("odd", (3, 2, 2)), ("big", (2, 1, 2))])
 def test_tile_fields_match_the_oracle(ctx, gname, tile):
     import torch
     from oracle import fields as OF
