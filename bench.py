@@ -215,8 +215,8 @@ def serve_stream(args, ct, tuned):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3")
     ap.add_argument("--form", default="auto", choices=["auto", "plain", "persistent"])
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-peak form")
@@ -345,6 +345,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
+        "step_ms": [round(w * 1e3, 3) for w in walls],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

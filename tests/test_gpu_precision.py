@@ -1,7 +1,9 @@
 """Catalogue precision against the FP64 oracle at the c4 resolution
-(alatt = 250/768 Mpc/h, 20 top-hat filters 1-15 Mpc/h, 2LPT, z = 0.5) in a
-256^3 box: collapse decisions, R_TH within one shell bin and M200m within
-0.5% for >= 99.5% of halos; every catalogue field's error is reported."""
+(alatt = 250/768 Mpc/h, 20 top-hat filters 1-15 Mpc/h, 2LPT, z = 0.5), in a
+256^3 box and on the full nmesh-768 tile: collapse decisions, R_TH within one
+shell bin and M200m within 0.5% for >= 99.5% of halos; every catalogue
+field's error is reported, for the FP32 fields of the product and for an
+FP16-stored variant."""
 import json
 
 import numpy as np
@@ -20,4 +22,20 @@ def test_c4_resolution_precision():
     assert f32["rthl_within_one_bin"] >= 0.995
     assert f32["mass_within_0.5pct"] >= 0.995
     # the FP16-stored variant is reported; acceptance is its one-bin rate
+    assert "accepted" in r["fp16_stored_fields"]
+
+
+def test_c4_full_size_precision():
+    """The c4 tile itself (nmesh 768, box 250 Mpc/h): every catalogue field of
+    both the product (FP32 fields) and the FP16-stored variant against the FP64
+    oracle over every peak."""
+    from precision_study import study
+    r = study("c4", n=None, fp16=True, threads=16)
+    print(json.dumps({k: v for k, v in r.items() if k != "fp16_stored_fields"}))
+    print(json.dumps({"fp16_stored_fields": {k: v for k, v in r["fp16_stored_fields"].items() if k != "fields"}}))
+    f32 = r["fp32_fields"]
+    assert f32["halos_oracle"] > 100_000
+    assert f32["status_agreement"] >= 0.995
+    assert f32["rthl_within_one_bin"] >= 0.995
+    assert f32["mass_within_0.5pct"] >= 0.995
     assert "accepted" in r["fp16_stored_fields"]
