@@ -20,6 +20,7 @@ OpenMP over peaks) timed on a bounded stratified sample of the same peaks.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -58,17 +59,25 @@ def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
         drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
     walls, kern, stages = [], [], []
     res = None
-    for _ in range(steps):
-        res = None  # the previous step's result arrays go back to the allocator before this step
-        mask.zero_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
-        torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
-        tm = ctx.last_timing()
-        kern.append(tm["shell_ms"] / 1e3)
-        stages.append(tm["stages_ms"])
+    # a collection of Python's cyclic garbage collector inside a timed step
+    # (tens of ms with torch loaded) would be timed as the step's; collect
+    # now and hold it off until the timed steps are done
+    gc.collect()
+    gc.disable()
+    try:
+        for _ in range(steps):
+            res = None  # the previous step's result arrays go back to the allocator before this step
+            mask.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+            tm = ctx.last_timing()
+            kern.append(tm["shell_ms"] / 1e3)
+            stages.append(tm["stages_ms"])
+    finally:
+        gc.enable()
     res = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in res.items()}
     return res, walls, kern, stages
 
