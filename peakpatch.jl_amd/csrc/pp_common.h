@@ -142,7 +142,7 @@ struct pp_ctx {
     unsigned long long fft_clock = 0;
     // scratch
     pp::DevBuf d_field_a, d_field_b, d_k0, d_k1, d_k2, d_mask, d_misc, d_peaks, d_scan, d_cub;
-    pp::DevBuf d_in[8], d_pk, d_res, d_prof, d_queue;
+    pp::DevBuf d_in[8], d_pk, d_res, d_prof, d_queue, d_walk;
     // persistent-form tunables (pp_autotune_shell): small-peak threshold in
     // finalised shells (several peaks per wave), shells per LDS chunk of the
     // wave gather, the shell count above which a peak's Float64 post runs on

@@ -217,7 +217,7 @@ int pp_ctx_destroy(pp_ctx *ctx)
     for (FftPlans &F : ctx->fft_cache) F.release();
     DevBuf *bufs[] = {&ctx->d_table, &ctx->d_field_a, &ctx->d_field_b, &ctx->d_k0, &ctx->d_k1,
                       &ctx->d_k2, &ctx->d_mask, &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan,
-                      &ctx->d_cub, &ctx->d_pk, &ctx->d_res, &ctx->d_prof, &ctx->d_queue,
+                      &ctx->d_cub, &ctx->d_pk, &ctx->d_res, &ctx->d_prof, &ctx->d_queue, &ctx->d_walk,
                       &ctx->shells.d_cell, &ctx->shells.d_shell_start, &ctx->shells.d_shell_r2,
                       &ctx->shells.d_shell_of_r2, &ctx->shells.d_groups, &ctx->shells.d_rad,
                       &ctx->shells.d_akk, &ctx->shells.d_wrow, &ctx->shells.d_wtab,
@@ -289,7 +289,7 @@ int pp_ctx_release_scratch(pp_ctx *ctx)
     PP_HIP(hipStreamSynchronize(ctx->side));
     DevBuf *bufs[] = {&ctx->d_field_a, &ctx->d_field_b, &ctx->d_k0, &ctx->d_k1, &ctx->d_k2, &ctx->d_mask,
                       &ctx->d_misc, &ctx->d_peaks, &ctx->d_scan, &ctx->d_cub, &ctx->d_pk, &ctx->d_res,
-                      &ctx->d_prof, &ctx->d_queue};
+                      &ctx->d_prof, &ctx->d_queue, &ctx->d_walk};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : ctx->d_in) b.release();
     return PP_OK;

@@ -320,12 +320,12 @@ __device__ __forceinline__ double wave_scan(double x)
 // batch by (peak, shell).  The lanes of each (peak, shell) run sum their
 // cells' 21 contributions by a segmented tree over wave shuffles (step d:
 // lane += lane + d while inside the run), and the run's first lane adds the
-// sum to that shell's LDS bin.  Records carry S, S2 and lapd so the lane
-// post can finish these peaks itself.
+// sum to that shell's LDS bin.  Records carry S, S2 and lapd, from which
+// k_small_finish forms these peaks' Sbar, Sbar2, d2F and mask.
 // A lane plays two roles per batch.  Loading, it reads the cell at its
 // position in the batch's row-major order (lcell, lq), so a load instruction
-// covers whole x-runs of the peaks' rows; its contributions go through LDS to
-// the lane at the cell's position in (peak, shell) order, which sums runs of
+// covers whole x-runs of the peaks' rows; its contributions go by a wave
+// shuffle to the lane at the cell's position in (peak, shell) order, which sums runs of
 // equal (peak, shell) with the tree (cell, q, s, run, rend; src = the lane
 // that loaded this position's cell).
 struct CellSlot {
@@ -371,7 +371,6 @@ __device__ __forceinline__ constexpr int sf_slot(int f) { return f < P_NB ? f : 
 __global__ void __launch_bounds__(64) k_small(SmallArgs A)
 {
     __shared__ float bins[NSF][64];  // (peak q, shell s) at q m + s - 1
-    __shared__ float stage[NSF][64]; // a batch's contributions, by loading lane
     __shared__ int4 spk[64];         // centre and flags of the block's peaks
     __shared__ double sdF[64], sfb[64];
     const int lane = threadIdx.x;
@@ -427,12 +426,9 @@ __global__ void __launch_bounds__(64) k_small(SmallArgs A)
                     if (F.lap) acc[P_NB + 6] = F.lap[a];
                 }
             }
-            // row-major loads -> (peak, shell) order
+            // row-major loads -> (peak, shell) order: lane takes lane src's sums
 #pragma unroll
-            for (int f = 0; f < NSF; f++) stage[f][lane] = acc[f];
-            __syncthreads();
-#pragma unroll
-            for (int f = 0; f < NSF; f++) acc[f] = stage[f][t.src];
+            for (int f = 0; f < NSF; f++) acc[f] = __shfl(acc[f], t.src, 64);
             for (int k = 0; k < S.steps; k++) {
                 const int d = 1 << k;
                 const bool in = lane + d < t.rend;
@@ -621,6 +617,8 @@ struct AosProfile {
     const float *se, *sg, *ax;  // this peak's records (E, G Gf, the rest)
     const double *radt;
     bool lapd;
+    const WalkRec *wt = nullptr;  // walk table: Ebar of shells 1..wmc (lane post of the walk-table class)
+    int wmc = 0;
     __device__ const float *x(int s) const { return ax + (int64_t)(s - 1) * PSR; }
     __device__ double rad(int s) const { return radt[s - 1]; }
     __device__ double Fbar(int s) const { return *(const double *)(x(s) + (P_FB - PSR)); }
@@ -685,6 +683,7 @@ constexpr int kSmallGridCU = 32;
 constexpr int kWaveGridCU = 16;
 constexpr int kFinishGridCU = 64;
 constexpr int kLaneGridCU = 64;
+constexpr int kWalkGridCU = 16;  // 256-lane workgroups per CU of the walk table (grid-stride over the peaks)
 
 // The lane post's order: m descending, then the fcrit crossing descending,
 // ties in position order (a stable radix sort of these keys), so the 64
@@ -692,16 +691,19 @@ constexpr int kLaneGridCU = 64;
 // through the same shell bands together.
 // keys[t - lo] = (cmax - m) << cbits | (cmax - crossing), m <= cmax < 2^cbits:
 // ascending keys give m and the crossing descending in 2 * cbits bits.
+// The crossing of every peak also goes to mc[t - lo]: the walk table holds
+// shells 1..mc of the peak.
 __global__ void k_crossing_keys(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
                                 const int64_t *__restrict__ prof_off, const double *__restrict__ rad,
-                                int cmax, int cbits, uint32_t *__restrict__ keys)
+                                int cmax, int cbits, uint32_t *__restrict__ keys, int64_t *__restrict__ mc)
 {
     for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += (int64_t)gridDim.x * blockDim.x) {
         const PeakParam pk = pps[t];
         const int64_t rb = prof_off[t];
         const AosProfile<false> P = profile_at<false>(prof, rb, rad, false);
-        const int mc = (pk.flags & 2) ? 0 : crossing_m0(P, pk);
-        keys[t - lo] = (uint32_t)(cmax - pk.m) << cbits | (uint32_t)(cmax - max(mc, 0));
+        const int c = (pk.flags & 2) ? 0 : max(crossing_m0(P, pk), 0);
+        keys[t - lo] = (uint32_t)(cmax - pk.m) << cbits | (uint32_t)(cmax - c);
+        mc[t - lo] = c;
     }
 }
 
@@ -711,6 +713,93 @@ __global__ void k_iota_from(int32_t *__restrict__ o, int64_t n, int32_t first)
         o[p] = first + (int32_t)p;
 }
 
+// The walk table: for a peak t in [lo, hi) with fcrit crossing mc(t) (the m0
+// its walk starts from), record mp - 1 at eoff(t) (eoff: exclusive prefix
+// sum of mc) holds the walk's step at shell mp: kernel_strain's Ebar over the
+// band walk_band gives at mp, and the step's zvir.  The lane post's walk then
+// takes one record per step instead of a band sum, an eigen-solve and a
+// table look-up, whose chains of dependent loads set the length of the
+// longest walks.  The records are formed as far as each walk goes: a group
+// of W lanes per peak (W by the peak's size, below) evaluates W consecutive
+// steps at a time from mc inward (lane l: shell top - l; the lanes' band
+// loads share cache lines), and the group stops after the first block
+// holding a step that ends the walk (a non-finite Ebar or zvir >= ZZon); the
+// steps below it in that block are written too and never read.  Each record
+// is the same arithmetic on the same inputs as the walk's own step.
+// Group widths by peak size: a walk is at most mc - 1 <= m - 1 steps and a
+// step's band sum grows with its radius, so larger peaks get wider groups
+// (fewer rounds of W steps on the longest walks).  Class k holds the peaks
+// with m >= kWalkClassM[k] (and below the previous class's bound).
+constexpr int kWalkClasses = 4;
+constexpr int kWalkClassW[kWalkClasses] = {64, 32, 16, 8};
+constexpr int kWalkClassM[kWalkClasses] = {256, 96, 48, 0};
+struct WalkClasses {
+    int64_t lo[kWalkClasses], hi[kWalkClasses];  // positions in the lane-post order, relative to its start
+    int32_t blk[kWalkClasses + 1];               // first workgroup of each class
+};
+
+template <int W>
+__device__ __forceinline__ void walk_group_class(const PeakParam *__restrict__ pps, int64_t lo, int64_t c_lo,
+                                                 int64_t c_hi, int64_t b0, int64_t nb,
+                                                 const int32_t *__restrict__ list, ProfBuf prof,
+                                                 const int64_t *__restrict__ prof_off, const PostConst &C,
+                                                 const int64_t *__restrict__ eoff, const int64_t *__restrict__ mc,
+                                                 WalkRec *__restrict__ out)
+{
+    const int lane = threadIdx.x & (W - 1);
+    const unsigned long long gmask = (W == 64 ? ~0ull : (1ull << W) - 1) << ((threadIdx.x & 63) & ~(W - 1));
+    const int64_t ngroups = nb * blockDim.x / W;
+    for (int64_t p = c_lo + ((blockIdx.x - b0) * (int64_t)blockDim.x + threadIdx.x) / W; p < c_hi; p += ngroups) {
+        const int64_t idx = list ? (int64_t)list[p] : lo + p;
+        const int64_t u = idx - lo;
+        const int c = (int)mc[u];
+        if (c == 0) continue;  // no crossing: no walk (the group's lanes hold the same peak)
+        const PeakParam pk = pps[idx];
+        const AosProfile<false> P = profile_at<false>(prof, prof_off[idx], C.rad, false);
+        WalkRec *rec = out + eoff[u];
+        for (int top = c; top >= 1; top -= W) {
+            const int mp = top - lane;
+            bool stop = false;
+            if (mp >= 1) {
+                int mlow, mupp;
+                walk_band(C, mp, pk.m, mlow, mupp);
+                const Strain sv = kernel_strain<1, KS_E>(P, C, mp, mlow, mupp);
+                WalkRec r;
+#pragma unroll
+                for (int q = 0; q < 6; q++) r.E[q] = sv.E[q];
+                const bool fin = finite6(sv.E);
+                double Fh, ev, pv;
+                r.zvir = fin ? step_zvir(C, sv.E, P.Fbar(mp), Fh, ev, pv) : -1.0;
+                const int e = mupp + 1;  // the first shell beyond the band
+                const bool edge = e <= pk.m && C.rad[e - 1] == C.rad[mp - 1] + 2.0;
+                r.mlow = mlow;
+                r.mupp = mupp | (edge ? 1 << 30 : 0);
+                rec[mp - 1] = r;
+                stop = mp < c && (!fin || r.zvir >= pk.ZZon);  // shell mc itself is the walk's start, not a step
+            }
+            if (__ballot(stop) & gmask) break;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_walk_group(const PeakParam *__restrict__ pps, int64_t lo,
+                                                    const int32_t *__restrict__ list, ProfBuf prof,
+                                                    const int64_t *__restrict__ prof_off, PostConst C,
+                                                    const int64_t *__restrict__ eoff, const int64_t *__restrict__ mc,
+                                                    WalkRec *__restrict__ out, WalkClasses K)
+{
+    int k = 0;
+    while (k < kWalkClasses - 1 && (int)blockIdx.x >= K.blk[k + 1]) k++;
+    const int64_t b0 = K.blk[k], nb = K.blk[k + 1] - K.blk[k];
+    if (nb <= 0) return;
+    switch (k) {
+    case 0: walk_group_class<kWalkClassW[0]>(pps, lo, K.lo[0], K.hi[0], b0, nb, list, prof, prof_off, C, eoff, mc, out); break;
+    case 1: walk_group_class<kWalkClassW[1]>(pps, lo, K.lo[1], K.hi[1], b0, nb, list, prof, prof_off, C, eoff, mc, out); break;
+    case 2: walk_group_class<kWalkClassW[2]>(pps, lo, K.lo[2], K.hi[2], b0, nb, list, prof, prof_off, C, eoff, mc, out); break;
+    default: walk_group_class<kWalkClassW[3]>(pps, lo, K.lo[3], K.hi[3], b0, nb, list, prof, prof_off, C, eoff, mc, out); break;
+    }
+}
+
 // One lane per peak.  A walk longer than max_steps (a wave runs in step with
 // its slowest lane) is handed to the wave post through the defer list.  With
 // a list, lane t takes the peak at list[t - lo].
@@ -718,19 +807,40 @@ template <bool FULL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1)))
 k_post_lane(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi, ProfBuf prof,
             const int64_t *__restrict__ prof_off, PostConst C, ResultView R, int has_lap, int max_steps,
-            int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer, const int32_t *__restrict__ list)
+            int64_t *__restrict__ defer, unsigned int *__restrict__ ndefer, const int32_t *__restrict__ list,
+            const WalkRec *__restrict__ wtab, const int64_t *__restrict__ eoff, const int64_t *__restrict__ wmc)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += stride) {
         const int64_t idx = list ? (int64_t)list[t - lo] : t;
         const PeakParam pk = pps[idx];
         const int64_t rb = prof_off[idx];
-        const AosProfile<FULL> P = profile_at<FULL>(prof, rb, C.rad, has_lap != 0);
-        if (analyse_post<1, FULL>(P, pk, C, R, pk.order, max_steps) == POST_DEFER) {
+        AosProfile<FULL> P = profile_at<FULL>(prof, rb, C.rad, has_lap != 0);
+        if (wtab) {  // the table covers [lo, hi)
+            P.wt = wtab + eoff[idx - lo];
+            P.wmc = (int)wmc[idx - lo];
+        }
+        if (analyse_post<1, false>(P, pk, C, R, pk.order, max_steps) == POST_DEFER) {
             defer[atomicAdd(ndefer, 1u)] = idx;
             continue;
         }
         R.nshells[pk.order] = pk.m;
+    }
+}
+
+// Phase 7's record sums for the collapsed small peaks (k_small's records
+// carry S, S2 and lapd per shell), one lane per peak, after their post: kept
+// out of the post, whose registers it would otherwise add to.
+__global__ void __launch_bounds__(256) k_small_finish(const PeakParam *__restrict__ pps, int64_t lo, int64_t hi,
+                                                      ProfBuf prof, const int64_t *__restrict__ prof_off,
+                                                      PostConst C, ResultView R, int has_lap)
+{
+    for (int64_t t = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += (int64_t)gridDim.x * blockDim.x) {
+        const PeakParam pk = pps[t];
+        const int64_t o = pk.order;
+        if (R.status[o] != DUMP_NONE) continue;
+        const AosProfile<true> P = profile_at<true>(prof, prof_off[t], C.rad, has_lap != 0);
+        finish_records<1>(P, pk, C, R, o, R.m0[o], R.q[o]);
     }
 }
 
@@ -1415,22 +1525,69 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     for (int x = std::min(std::max(ctx->coop_m, 1), nb - 1); x > small_m; x--)
         if (hist[x]) { cmax = x; break; }
     while ((1 << cbits) <= cmax) cbits++;
+    // The walk table of the same class (k_walk_strain): per peak its crossing
+    // mc and the prefix sum of mc, then one record per shell 1..mc; sized for
+    // the class's shells (mc <= m), known here from the m histogram.
+    int64_t *d_wmc = nullptr, *d_weoff = nullptr;
+    WalkRec *d_wtab = nullptr;
+    int64_t wrec_max = 0;
+    for (int x = small_m + 1; x <= cmax && x < nb; x++) wrec_max += (int64_t)hist[x] * x;
     auto cross_order = [&]() -> int {
         const int64_t nl = cross_hi - cross_lo;
         if (ctx->post_width > 1 || nl <= 64) return PP_OK;
+        PP_CHECK(ctx->d_walk.grow(al((size_t)nl * 8) * 2 + (size_t)std::max<int64_t>(wrec_max, 1) * sizeof(WalkRec)));
+        d_wmc = ctx->d_walk.as<int64_t>();
+        d_weoff = (int64_t *)(ctx->d_walk.as<char>() + al((size_t)nl * 8));
+        d_wtab = (WalkRec *)(ctx->d_walk.as<char>() + 2 * al((size_t)nl * 8));
         uint32_t *ka = (uint32_t *)W.ka, *kb = (uint32_t *)W.kb;
         hipLaunchKernelGGL(k_crossing_keys, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, (const PeakParam *)pps,
-                           cross_lo, cross_hi, d_rec, (const int64_t *)d_off, C.rad, cmax, cbits, ka);
+                           cross_lo, cross_hi, d_rec, (const int64_t *)d_off, C.rad, cmax, cbits, ka, d_wmc);
         hipLaunchKernelGGL(k_iota_from, dim3(grid1d(ctx, nl)), dim3(256), 0, ctx->stream, W.iota, nl, (int32_t)cross_lo);
         PP_HIP(hipGetLastError());
         // stable: equal keys keep the caller's order (neighbouring profiles and result rows)
-        size_t tmp = 0;
+        size_t tmp = 0, tmp2 = 0;
         PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 2 * cbits,
                                                   ctx->stream));
-        PP_CHECK(ctx->d_cub.grow(tmp));
+        PP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, d_wmc, d_weoff, (int)nl, ctx->stream));
+        PP_CHECK(ctx->d_cub.grow(std::max(tmp, tmp2)));
         PP_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub.ptr, tmp, ka, kb, W.iota, W.mo, (int)nl, 0, 2 * cbits,
                                                   ctx->stream));
+        PP_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_cub.ptr, tmp2, d_wmc, d_weoff, (int)nl, ctx->stream));
         cross_list = W.mo;
+        return PP_OK;
+    };
+    // the walk table's records (after the gather's profiles and the crossings)
+    auto walk_table = [&]() -> int {
+        if (!d_wtab) return PP_OK;
+        // classes: consecutive runs of the lane-post order (m descending)
+        WalkClasses K{};
+        int64_t want[kWalkClasses], tot = 0, pos = 0;
+        for (int k = 0; k < kWalkClasses; k++) {
+            const int mlo = std::max(kWalkClassM[k], small_m + 1);
+            const int mhi = k == 0 ? cmax : kWalkClassM[k - 1] - 1;
+            int64_t cnt = 0;
+            for (int x = mlo; x <= std::min(mhi, cmax) && x < nb; x++) cnt += hist[x];
+            K.lo[k] = pos;
+            K.hi[k] = pos += cnt;
+            want[k] = (cnt * kWalkClassW[k] + 255) / 256;
+            tot += want[k];
+        }
+        if (pos != cross_hi - cross_lo) {
+            set_error("walk classes hold %lld of %lld peaks", (long long)pos, (long long)(cross_hi - cross_lo));
+            return PP_ERR_STATE;
+        }
+        const int64_t cap = (int64_t)ctx->num_cu * kWalkGridCU;
+        K.blk[0] = 0;
+        for (int k = 0; k < kWalkClasses; k++) {
+            int64_t b = want[k];
+            if (tot > cap && b > 0) b = std::max<int64_t>(1, b * cap / tot);
+            K.blk[k + 1] = K.blk[k] + (int32_t)b;
+        }
+        if (K.blk[kWalkClasses] == 0) return PP_OK;
+        hipLaunchKernelGGL(k_walk_group, dim3(K.blk[kWalkClasses]), dim3(256), 0, ctx->stream,
+                           (const PeakParam *)pps, cross_lo, cross_list, d_rec, (const int64_t *)d_off, C,
+                           (const int64_t *)d_weoff, (const int64_t *)d_wmc, d_wtab, K);
+        PP_HIP(hipGetLastError());
         return PP_OK;
     };
     auto post_lane = [&](int64_t lo, int64_t hi, bool full) -> int {
@@ -1440,15 +1597,19 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         if (full) {
             hipLaunchKernelGGL(k_post_lane<true>, dim3(gp), dim3(64), 0, ctx->side, (const PeakParam *)pps, lo, hi,
                                d_rec, (const int64_t *)d_off, C, S.R, G.lap != nullptr ? 1 : 0, 0x7fffffff, d_defer,
-                               d_ndefer, (const int32_t *)nullptr);
+                               d_ndefer, (const int32_t *)nullptr, (const WalkRec *)nullptr,
+                               (const int64_t *)nullptr, (const int64_t *)nullptr);
         } else {
             const int budget = ctx->lane_walk > 0 ? ctx->lane_walk : 0x7fffffff;
             hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, (int *)d_ndefer, (int64_t)1, 0);
             // lanes of a wave start at the same crossing where possible (the
             // order is built before the fork, see cross_order)
-            const int32_t *list = (lo == cross_lo && hi == cross_hi) ? cross_list : nullptr;
+            const bool cross = lo == cross_lo && hi == cross_hi;
+            const int32_t *list = cross ? cross_list : nullptr;
             hipLaunchKernelGGL(k_post_lane<false>, dim3(gp), dim3(64), 0, ctx->stream, (const PeakParam *)pps, lo, hi,
-                               d_rec, (const int64_t *)d_off, C, S.R, 0, budget, d_defer, d_ndefer, list);
+                               d_rec, (const int64_t *)d_off, C, S.R, 0, budget, d_defer, d_ndefer, list,
+                               (const WalkRec *)(cross ? d_wtab : nullptr), (const int64_t *)d_weoff,
+                               (const int64_t *)d_wmc);
             if (ctx->lane_walk > 0) {
                 const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(hi - lo, (int64_t)ctx->num_cu * 16));
                 hipLaunchKernelGGL(k_post_defer, dim3(gw), dim3(64), 0, ctx->stream, (const PeakParam *)pps,
@@ -1497,6 +1658,12 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         }
         PP_HIP(hipEventRecord(ctx->ev[9], ctx->side));
         PP_CHECK(post_lane(small_lo, np, true));
+        if (np > small_lo) {
+            hipLaunchKernelGGL(k_small_finish, dim3(grid1d(ctx, np - small_lo)), dim3(256), 0, ctx->side,
+                               (const PeakParam *)pps, small_lo, np, d_rec, (const int64_t *)d_off, C, S.R,
+                               G.lap != nullptr ? 1 : 0);
+            PP_HIP(hipGetLastError());
+        }
         PP_HIP(hipEventRecord(ctx->ev[14], ctx->side));
         return PP_OK;
     };
@@ -1530,6 +1697,7 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
     // peaks' work), beside the lane post on the main stream
     PP_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
     if (ctx->post_width <= 1) {
+        PP_CHECK(walk_table());
         PP_CHECK(post_lane(coop_end, small_lo, false));
     } else if (small_lo > coop_end) {
         const int W = ctx->post_width;

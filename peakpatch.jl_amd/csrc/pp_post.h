@@ -357,6 +357,92 @@ __device__ __forceinline__ void evals_ep(const double E[6], double &Frho, double
     }
 }
 
+// One walk step computed ahead of the walk (the persistent form's walk
+// table): kernel_strain's scaled Ebar at shell mp over the band
+// [mlow, mupp & kWalkBand] it was formed on, and the step's zvir (the
+// collapse-table value the walk compares with ZZon; Fbar(mp) until the
+// table's second pass replaces it).  Bit 30 of mupp is set
+// when shell mupp + 1 lies at exactly rad(mp) + 2: a walk that arrives with
+// that shell as its band top keeps it (the reference's test is rad > rupp).
+// 64 bytes: four 16-byte loads.
+constexpr int32_t kWalkBand = (1 << 30) - 1;
+struct WalkRec {
+    double E[6];
+    double zvir;
+    int32_t mlow, mupp;
+};
+
+// The step's eigenvalue measures and zvir from its Ebar and Fbar(mp)
+// (RadialShell.jl:600-640).
+__device__ __forceinline__ double step_zvir(const PostConst &C, const double E[6], double Fb, double &Frhoh,
+                                            double &e_v, double &p_v)
+{
+    evals_ep(E, Frhoh, e_v, p_v);
+    double z = -1.0;
+    if (Fb > 0.0) {
+        const double poe = e_v < 1e-5 ? 0.0 : p_v / e_v;
+        z = d_interp(C.tab, log10(Fb), e_v, poe);
+    }
+    return z;
+}
+
+// The band a walk step at mp takes when the walk reaches mp from outside:
+// shells with rad in (rad(mp) - 2, rad(mp) + 2), at most m (Phase 5,
+// RadialShell.jl:540-600).  The walk's own lower edge is always this one (at
+// mp = 1 it keeps shell 2, the edge of every step above it: the centre is
+// never inside rad > 0); its upper edge differs only where a shell lies at
+// exactly rad(mp) + 2.
+__device__ __forceinline__ void walk_band(const PostConst &C, int mp, int m, int &mlow, int &mupp)
+{
+    const double r = C.rad[mp - 1];
+    mlow = mp > 1 ? mp : 2;
+    if (mp > 1) {
+        const int a = first_rad_gt(C, 1, mp - 1, fmax(r - 2.0, 0.0));
+        if (a <= mp - 1) mlow = a;
+    }
+    const int b = last_rad_lt(C, mp + 1, m, r + 2.0);
+    mupp = b >= mp + 1 ? b : mp;
+}
+
+template <class PV>
+__device__ __forceinline__ const WalkRec &tab_rec(const PV &P, int mp)
+{
+    return P.wt[mp - 1];
+}
+template <class PV>
+__device__ __forceinline__ int tab_crossing(const PV &P)
+{
+    return P.wmc;
+}
+
+// Ebar of a walk step: the peak's walk-table record for mp when it was formed
+// on the same band (identical to kernel_strain's, which computed it), else
+// kernel_strain itself.  Only the one-lane-per-peak post has a table.
+template <int W, class PV>
+__device__ __forceinline__ Strain walk_strain(const PV &P, const PostConst &C, int mp, int mlow, int mupp)
+{
+    if constexpr (W == 1) {
+        if (P.wt && mp <= P.wmc) {
+            const WalkRec *r = P.wt + (mp - 1);
+            const int2 band = *(const int2 *)&r->mlow;
+            if (band.x == mlow && (band.y & kWalkBand) == mupp) {
+                Strain s;
+                const double2 *e = (const double2 *)r->E;
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const double2 v = e[q];
+                    s.E[2 * q] = v.x;
+                    s.E[2 * q + 1] = v.y;
+                }
+#pragma unroll
+                for (int L = 0; L < 3; L++) s.g[L] = s.gf[L] = 0.0;
+                return s;
+            }
+        }
+    }
+    return kernel_strain<W, KS_E>(P, C, mp, mlow, mupp);
+}
+
 __device__ __forceinline__ void write_nocollapse(const ResultView &R, int64_t o, int status)
 {
     R.RTHL[o] = -1.0; R.Srb[o] = 0.0; R.Fbarx[o] = 0.0; R.e_v[o] = 0.0; R.p_v[o] = 0.0;
@@ -444,6 +530,73 @@ __device__ void finish_cells(const PostConst &C, const PeakParam &pk, int m0, in
     }
 }
 
+// Phase 7's sums over the shells inside the collapse (RadialShell.jl:756-819)
+// from a peak's shell records: Sbar and Sbar2 over shells 1..m0-1, d2F over
+// the cells of shells 2..q with the stop at the first out-of-bounds cell,
+// and the mask over the cells of shells 1..q.  Inline in the post when its
+// profile carries the S, S2 and lapd sums; the persistent form's small peaks
+// run it in a pass of its own after their post (k_small_finish).
+template <int W, class PV>
+__device__ void finish_records(const PV &P, const PeakParam &pk, const PostConst &C, const ResultView &R,
+                               int64_t o, int m0, int q)
+{
+    using X = Exec<W>;
+    const int lane = X::lane();
+    double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
+    for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += W) {
+        double cnt, Sv[3], S2v[3];
+        P.sums(m1, cnt, Sv, S2v);
+        if (cnt > 0.0) {
+            for (int L = 0; L < 3; L++) { Sb[L] += Sv[L]; Sb2[L] += S2v[L]; }
+            nS += cnt;
+        }
+    }
+    for (int L = 0; L < 3; L++) { Sb[L] = X::sum(Sb[L]); Sb2[L] = X::sum(Sb2[L]); }
+    nS = X::sum(nS);
+    if (nS > 0.0) for (int L = 0; L < 3; L++) { Sb[L] /= nS; Sb2[L] /= nS; }
+    const int cend = min(C.shell_start[q], pk.npart);
+    const int Rq = (int)floor(P.rad(q));
+    const bool inside_q = pk.ci - Rq >= 0 && pk.ci + Rq < C.n1 && pk.cj - Rq >= 0 && pk.cj + Rq < C.n2 &&
+                          pk.ck - Rq >= 0 && pk.ck + Rq < C.n3;
+    if (C.mask) {
+        const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
+        const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
+        for (int c = lane; c < cend; c += W) {
+            int pc = C.cell[c];
+            int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+            if (x < lo1 || x >= hi1 || y < lo2 || y >= hi2 || z < lo3 || z >= hi3) continue;
+            C.mask[x + (long)y * C.n1 + (long)z * C.n1 * C.n2] = 1;
+        }
+    }
+    double d2F = 0.0;
+    if (P.has_lapd()) {
+        if (inside_q) {
+            double sL = 0.0, sn = 0.0;
+            for (int m1 = 2 + lane; m1 <= q; m1 += W) { sL += P.lap(m1); sn += P.count(m1); }
+            sL = X::sum(sL);
+            sn = X::sum(sn);
+            if (sn > 0.0) d2F = sL / sn;
+        } else if (X::leader()) {
+            // RadialShell.jl:804-819: stop at the first out-of-bounds cell
+            double sacc = 0.0;
+            long nav = 0;
+            for (int c = 1; c < cend; c++) {
+                int pc = C.cell[c];
+                int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
+                if (x < 0 || x >= C.n1 || y < 0 || y >= C.n2 || z < 0 || z >= C.n3) break;
+                nav++;
+                sacc += (double)C.lapd[x + (long)y * C.n1 + (long)z * C.n1 * C.n2];
+            }
+            d2F = nav > 0 ? sacc / (double)nav : 0.0;
+        }
+        if constexpr (W > 1) d2F = __shfl(d2F, X::base(), 64);
+    }
+    if (X::leader()) {
+        for (int L = 0; L < 3; L++) { R.Sbar[o * 3 + L] = Sb[L]; R.Sbar2[o * 3 + L] = Sb2[L]; }
+        R.d2F[o] = d2F;
+    }
+}
+
 // The fcrit crossing of one peak, serially (RadialShell.jl:492-538): the m0
 // the inward walk starts from, as analyse_post<1> finds it; 0 when there is
 // none (a DUMP_NO_FCRIT peak).  Used to order the lane post's peaks.
@@ -496,7 +649,18 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         // ---- Phase 4: fcrit crossing (RadialShell.jl:492-538) ----
         const double fcrit = pk.fcrit;
         int m0 = pk.m0, mupp;
-        if (P.Fbar(m0) >= fcrit) {
+        bool tab = false;  // the walk table (one lane per peak): crossing, bands and steps from it
+        if constexpr (W == 1) tab = P.wt != nullptr;
+        if (tab) {
+            // the crossing crossing_m0 found ahead of the walk (0: none); an
+            // outward search that found none (the crossing is pk.m0 itself)
+            // keeps the band top at m
+            const int mc = tab_crossing(P);
+            if (mc == 0) { status = DUMP_NO_FCRIT; goto nocol; }
+            const bool outward_none = mc == m0 && P.Fbar(m0) >= fcrit;
+            m0 = mc;
+            mupp = outward_none ? m : (tab_rec(P, m0).mupp & kWalkBand);
+        } else if (P.Fbar(m0) >= fcrit) {
             int found = 0;
             if constexpr (W > 1) {
                 for (int base = m0; base <= m; base += W) {
@@ -536,11 +700,16 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         }
 
         // ---- Phase 5: inward walk (RadialShell.jl:540-704) ----
-        double rlow = fmax(P.rad(m0) - 2.0, 0.0);
         int mlow = m0;
-        if (m0 > 1) { int a = first_rad_gt(C, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
+        if (tab) {
+            // walk_band's edge, except at m0 = 1, where the start keeps the centre
+            if (m0 > 1) mlow = tab_rec(P, m0).mlow;
+        } else {
+            const double rlow = fmax(P.rad(m0) - 2.0, 0.0);
+            if (m0 > 1) { int a = first_rad_gt(C, 1, m0 - 1, rlow); if (a <= m0 - 1) mlow = a; }
+        }
         const double ZZon = pk.ZZon;
-        Strain s0 = kernel_strain<W, KS_E>(P, C, m0, mlow, mupp);
+        Strain s0 = walk_strain<W>(P, C, m0, mlow, mupp);
         if (!finite6(s0.E)) { status = DUMP_NONFINITE; goto nocol; }
         double Frho_m0, e_v_m0, p_v_m0;
         evals_ep(s0.E, Frho_m0, e_v_m0, p_v_m0);
@@ -563,40 +732,61 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
             for (int mp = m0 - 1; mp >= 1; mp--) {
                 // a caller with a step budget re-runs long walks elsewhere
                 if (m0 - 1 - mp >= max_steps) return POST_DEFER;
-                double rupp_mp = P.rad(mp) + 2.0;
-                double rlow_mp = fmax(P.rad(mp) - 2.0, 0.0);
-                if (P.rad(mupp) > rupp_mp) {
-                    int b = last_rad_lt(C, mp + 1, mupp, rupp_mp);
-                    mupp = b >= mp + 1 ? b : mp;
+                double E[6], zv, Frhoh_mp = 0.0, e_v_mp = 0.0, p_v_mp = 0.0;
+                bool from_tab = false;
+                if (tab) {
+                    // The table's step at mp was formed on the band this
+                    // walk takes at mp (walk_band): the lower edge always,
+                    // the upper edge unless the walk arrives with the shell
+                    // at exactly rad(mp) + 2 as its band top, which it then
+                    // keeps.  The table holds the steps down to its own first
+                    // collapse, so after such a step the walk goes on without
+                    // it.
+                    const WalkRec &r = tab_rec(P, mp);
+                    const int mu = r.mupp & kWalkBand;
+                    mupp = ((r.mupp >> 30) & 1) && mupp == mu + 1 ? mupp : mu;
+                    mlow = r.mlow;
+                    if (mupp == mu) {
+#pragma unroll
+                        for (int i = 0; i < 6; i++) E[i] = r.E[i];
+                        zv = r.zvir;
+                        from_tab = true;
+                    } else {
+                        tab = false;
+                    }
+                } else {
+                    const double rupp_mp = P.rad(mp) + 2.0;
+                    const double rlow_mp = fmax(P.rad(mp) - 2.0, 0.0);
+                    if (P.rad(mupp) > rupp_mp) {
+                        int b = last_rad_lt(C, mp + 1, mupp, rupp_mp);
+                        mupp = b >= mp + 1 ? b : mp;
+                    }
+                    if (mlow > 1 && P.rad(mlow - 1) > rlow_mp) {
+                        int a = first_rad_gt(C, 1, mlow - 1, rlow_mp);
+                        mlow = a <= mlow - 1 ? a : mlow;
+                    }
                 }
-                if (mlow > 1 && P.rad(mlow - 1) > rlow_mp) {
-                    int a = first_rad_gt(C, 1, mlow - 1, rlow_mp);
-                    mlow = a <= mlow - 1 ? a : mlow;
+                if (!from_tab) {
+                    const Strain smp = kernel_strain<W, KS_E>(P, C, mp, mlow, mupp);
+#pragma unroll
+                    for (int i = 0; i < 6; i++) E[i] = smp.E[i];
+                    if (!finite6(E)) { status = DUMP_NONFINITE; goto nocol; }
+                    zv = step_zvir(C, E, P.Fbar(mp), Frhoh_mp, e_v_mp, p_v_mp);
                 }
-                Strain smp = kernel_strain<W, KS_E>(P, C, mp, mlow, mupp);
-                if (!finite6(smp.E)) { status = DUMP_NONFINITE; goto nocol; }
-                double Frho_mp, e_v_mp, p_v_mp;
-                evals_ep(smp.E, Frho_mp, e_v_mp, p_v_mp);
-                double Frhoh_mp = Frho_mp;
-                Frho_mp = P.Fbar(mp);
-                double zvir1p_mp = -1.0;
-                if (Frho_mp > 0.0) {
-                    double poe = e_v_mp < 1e-5 ? 0.0 : p_v_mp / e_v_mp;
-                    zvir1p_mp = d_interp(C.tab, log10(Frho_mp), e_v_mp, poe);
-                }
-                if (zvir1p_mp >= ZZon) {
+                if (!finite6(E)) { status = DUMP_NONFINITE; goto nocol; }
+                if (zv >= ZZon) {
                     m0 = mp + 1;
-                    zvir1p = zvir1p_mp;
-                    for (int i = 0; i < 6; i++) lastE[i] = smp.E[i];
+                    zvir1p = zv;
+                    for (int i = 0; i < 6; i++) lastE[i] = E[i];
                     band_last[0] = mp; band_last[1] = mlow; band_last[2] = mupp;
-                    Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
+                    if (from_tab) evals_ep(E, Frhoh_mp, e_v_mp, p_v_mp);
+                    Frhoh = Frhoh_mp; Frho = P.Fbar(mp); e_v = e_v_mp; p_v = p_v_mp;
                     collapsed = true;
                     break;
                 }
-                zvir1 = zvir1p_mp;
+                zvir1 = zv;
                 band_prev[0] = mp; band_prev[1] = mlow; band_prev[2] = mupp;
-                for (int i = 0; i < 6; i++) strain_mat[i] = smp.E[i];
-                Frhoh = Frhoh_mp; Frho = Frho_mp; e_v = e_v_mp; p_v = p_v_mp;
+                for (int i = 0; i < 6; i++) strain_mat[i] = E[i];
             }
         }
         (void)Frho;
@@ -679,61 +869,7 @@ __device__ int analyse_post(const PV &P, const PeakParam &pk, const PostConst &C
         // cells with r <= RTHL: shells 1..q
         const int q = last_rad_lt(C, 1, m, nextafter(RTHL, 1e300));
 
-        if constexpr (FINISH) {
-            double Sb[3] = {0, 0, 0}, Sb2[3] = {0, 0, 0}, nS = 0.0;
-            for (int m1 = 1 + lane; m1 <= m0 - 1; m1 += W) {
-                double cnt, Sv[3], S2v[3];
-                P.sums(m1, cnt, Sv, S2v);
-                if (cnt > 0.0) {
-                    for (int L = 0; L < 3; L++) { Sb[L] += Sv[L]; Sb2[L] += S2v[L]; }
-                    nS += cnt;
-                }
-            }
-            for (int L = 0; L < 3; L++) { Sb[L] = X::sum(Sb[L]); Sb2[L] = X::sum(Sb2[L]); }
-            nS = X::sum(nS);
-            if (nS > 0.0) for (int L = 0; L < 3; L++) { Sb[L] /= nS; Sb2[L] /= nS; }
-            const int cend = min(C.shell_start[q], pk.npart);
-            const int Rq = (int)floor(P.rad(q));
-            const bool inside_q = pk.ci - Rq >= 0 && pk.ci + Rq < C.n1 && pk.cj - Rq >= 0 && pk.cj + Rq < C.n2 &&
-                                  pk.ck - Rq >= 0 && pk.ck + Rq < C.n3;
-            if (C.mask) {
-                const int lo1 = C.nbuff, hi1 = C.n1 - C.nbuff, lo2 = C.nbuff, hi2 = C.n2 - C.nbuff;
-                const int lo3 = C.nbuff, hi3 = C.n3 - C.nbuff;
-                for (int c = lane; c < cend; c += W) {
-                    int pc = C.cell[c];
-                    int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
-                    if (x < lo1 || x >= hi1 || y < lo2 || y >= hi2 || z < lo3 || z >= hi3) continue;
-                    C.mask[x + (long)y * C.n1 + (long)z * C.n1 * C.n2] = 1;
-                }
-            }
-            double d2F = 0.0;
-            if (P.has_lapd()) {
-                if (inside_q) {
-                    double sL = 0.0, sn = 0.0;
-                    for (int m1 = 2 + lane; m1 <= q; m1 += W) { sL += P.lap(m1); sn += P.count(m1); }
-                    sL = X::sum(sL);
-                    sn = X::sum(sn);
-                    if (sn > 0.0) d2F = sL / sn;
-                } else if (X::leader()) {
-                    // RadialShell.jl:804-819: stop at the first out-of-bounds cell
-                    double sacc = 0.0;
-                    long nav = 0;
-                    for (int c = 1; c < cend; c++) {
-                        int pc = C.cell[c];
-                        int x = pk.ci + (pc & 255) - 128, y = pk.cj + ((pc >> 8) & 255) - 128, z = pk.ck + ((pc >> 16) & 255) - 128;
-                        if (x < 0 || x >= C.n1 || y < 0 || y >= C.n2 || z < 0 || z >= C.n3) break;
-                        nav++;
-                        sacc += (double)C.lapd[x + (long)y * C.n1 + (long)z * C.n1 * C.n2];
-                    }
-                    d2F = nav > 0 ? sacc / (double)nav : 0.0;
-                }
-                if constexpr (W > 1) d2F = __shfl(d2F, X::base(), 64);
-            }
-            if (X::leader()) {
-                for (int L = 0; L < 3; L++) { R.Sbar[o * 3 + L] = Sb[L]; R.Sbar2[o * 3 + L] = Sb2[L]; }
-                R.d2F[o] = d2F;
-            }
-        }
+        if constexpr (FINISH) finish_records<W>(P, pk, C, R, o, m0, q);
 
         // ---- zvir_half (RadialShell.jl:821-885) ----
         double zvir_half = -1.0;

@@ -117,6 +117,8 @@ struct GlobalProfile {
     const double *radt;
     int m;
     bool lapd;
+    const WalkRec *wt = nullptr;  // no walk table in this form
+    int wmc = 0;
     __device__ double rad(int s) const { return radt[s - 1]; }
     __device__ double Fbar(int s) const { return fb[s - 1]; }
     __device__ double count(int s) const { return (double)f[PF_N * m + s - 1]; }
