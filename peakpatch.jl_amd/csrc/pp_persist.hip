@@ -728,10 +728,13 @@ __global__ void k_iota_from(int32_t *__restrict__ o, int64_t n, int32_t first)
 // is the same arithmetic on the same inputs as the walk's own step.
 // Group widths by peak size: a walk is at most mc - 1 <= m - 1 steps and a
 // step's band sum grows with its radius, so larger peaks get wider groups
-// (fewer rounds of W steps on the longest walks).  Class k holds the peaks
-// with m >= kWalkClassM[k] (and below the previous class's bound).
+// (fewer rounds of W steps on the longest walks); narrower groups waste fewer
+// steps below a walk's end (measured at c3: widths 64/32/16/8 15.9 ms,
+// 16/8/4/4 15.74, 8/4/2/2 15.80, 4/2/2/1 16.0 for the whole call).  Class k
+// holds the peaks with m >= kWalkClassM[k] (and below the previous class's
+// bound).
 constexpr int kWalkClasses = 4;
-constexpr int kWalkClassW[kWalkClasses] = {64, 32, 16, 8};
+constexpr int kWalkClassW[kWalkClasses] = {16, 8, 4, 4};
 constexpr int kWalkClassM[kWalkClasses] = {256, 96, 48, 0};
 struct WalkClasses {
     int64_t lo[kWalkClasses], hi[kWalkClasses];  // positions in the lane-post order, relative to its start
