@@ -111,41 +111,42 @@ __global__ void __launch_bounds__(256) k_pack4(const float *__restrict__ d, cons
 }
 
 // ---------------------------------------------------------- ordering
-__global__ void k_keys(const PeakParam *__restrict__ pp, int64_t np, int32_t *__restrict__ keys, int *__restrict__ unsorted)
-{
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        const int m = pp[p].m;
-        keys[p] = m;
-        if (p + 1 < np && m < pp[p + 1].m) atomicOr(unsorted, 1);
-    }
-}
-
 __global__ void k_iota(int32_t *__restrict__ o, int64_t np)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
         o[p] = (int32_t)p;
 }
 
-__global__ void k_permute(const PeakParam *__restrict__ pp, const int32_t *__restrict__ order, int64_t np,
-                          PeakParam *__restrict__ out, int32_t *__restrict__ m_sorted)
-{
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        const PeakParam q = pp[order[p]];
-        out[p] = q;
-        m_sorted[p] = q.m;
-    }
-}
-
-__global__ void k_hist_m(const int32_t *__restrict__ m, int64_t np, int nbins, int *__restrict__ hist)
+// keys (m), the out-of-order flag and the histogram of m in one pass
+// (flag[0]: some peak has a larger m than the one before it)
+__global__ void k_keys_hist(const PeakParam *__restrict__ pp, int64_t np, int32_t *__restrict__ keys, int nbins,
+                            int *__restrict__ hist, int *__restrict__ flag)
 {
     extern __shared__ int h[];
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) h[b] = 0;
     __syncthreads();
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&h[min(m[p], nbins - 1)], 1);
+    bool up = false;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const int m = pp[p].m;
+        keys[p] = m;
+        up = up || (p + 1 < np && m < pp[p + 1].m);
+        atomicAdd(&h[min(m, nbins - 1)], 1);
+    }
+    if (up) atomicOr(flag, 1);
     __syncthreads();
     for (int b = threadIdx.x; b < nbins; b += blockDim.x)
         if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+// order: the sorted positions, or null when the batch is already in order
+__global__ void k_permute(const PeakParam *__restrict__ pp, const int32_t *__restrict__ order, int64_t np,
+                          PeakParam *__restrict__ out, int32_t *__restrict__ m_sorted)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const PeakParam q = pp[order ? order[p] : p];
+        out[p] = q;
+        m_sorted[p] = q.m;
+    }
 }
 
 __global__ void k_m_to_i64(const int32_t *__restrict__ m, int64_t n, int64_t *__restrict__ out)
@@ -1279,17 +1280,21 @@ static int order_peaks(pp_ctx *ctx, ShellRun &S, PeakParam *&pps, int32_t *&m_so
     W.mo = (int32_t *)p; p += al(np * 4);
     W.ka = (uint64_t *)p; p += al(np * 8);
     W.kb = (uint64_t *)p; p += al(np * 8);
-    int *d_hist = (int *)p; p += al(nb * 4);
-    int *d_flag = (int *)p;
-    hipLaunchKernelGGL(k_fill_i32b, dim3(1), dim3(64), 0, ctx->stream, d_flag, (int64_t)1, 0);
-    hipLaunchKernelGGL(k_keys, dim3(grid1d(ctx, np)), dim3(256), 0, ctx->stream, (const PeakParam *)S.pp, np, S.keys, d_flag);
-    hipLaunchKernelGGL(k_iota, dim3(grid1d(ctx, np)), dim3(256), 0, ctx->stream, S.order, np);
+    int *d_hist = (int *)p;  // nb bins, then the out-of-order flag
+    p += al((nb + 1) * 4);
+    // one pass for the keys, the flag and the histogram of m; one read-back
+    hipLaunchKernelGGL(k_fill_i32b, dim3(grid1d(ctx, nb + 1)), dim3(256), 0, ctx->stream, d_hist, (int64_t)nb + 1, 0);
+    hipLaunchKernelGGL(k_keys_hist, dim3(grid1d(ctx, np, 4)), dim3(256), nb * sizeof(int), ctx->stream,
+                       (const PeakParam *)S.pp, np, S.keys, nb, d_hist, d_hist + nb);
     PP_HIP(hipGetLastError());
-    int unsorted = 0;
-    PP_HIP(hipMemcpyAsync(&unsorted, d_flag, 4, hipMemcpyDeviceToHost, ctx->stream));
+    hist.assign(nb + 1, 0);
+    PP_HIP(hipMemcpyAsync(hist.data(), d_hist, (nb + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
-    const int32_t *order = S.order;
+    const int unsorted = hist[nb];
+    hist.resize(nb);
+    const int32_t *order = nullptr;
     if (unsorted) {
+        hipLaunchKernelGGL(k_iota, dim3(grid1d(ctx, np)), dim3(256), 0, ctx->stream, S.order, np);
         size_t tmp = 0;
         PP_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, S.keys, keys2, S.order, order2, (int)np, 0, 16,
                                                             ctx->stream));
@@ -1300,13 +1305,7 @@ static int order_peaks(pp_ctx *ctx, ShellRun &S, PeakParam *&pps, int32_t *&m_so
     }
     hipLaunchKernelGGL(k_permute, dim3(grid1d(ctx, np)), dim3(256), 0, ctx->stream, (const PeakParam *)S.pp, order, np,
                        pps, m_sorted);
-    hipLaunchKernelGGL(k_fill_i32b, dim3(grid1d(ctx, nb)), dim3(256), 0, ctx->stream, d_hist, (int64_t)nb, 0);
-    hipLaunchKernelGGL(k_hist_m, dim3(grid1d(ctx, np, 4)), dim3(256), nb * sizeof(int), ctx->stream,
-                       (const int32_t *)m_sorted, np, nb, d_hist);
     PP_HIP(hipGetLastError());
-    hist.assign(nb, 0);
-    PP_HIP(hipMemcpyAsync(hist.data(), d_hist, nb * 4, hipMemcpyDeviceToHost, ctx->stream));
-    PP_HIP(hipStreamSynchronize(ctx->stream));
     return PP_OK;
 }
 
@@ -1401,11 +1400,8 @@ static int run_persistent_impl(pp_ctx *ctx, GatherFields G, ShellRun &S, PostCon
         PP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_m64, d_off, (int)np, ctx->stream));
         PP_CHECK(ctx->d_cub.grow(tmp));
         PP_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_cub.ptr, tmp, d_m64, d_off, (int)np, ctx->stream));
-        int64_t last[2];
-        PP_HIP(hipMemcpyAsync(&last[0], d_off + np - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-        PP_HIP(hipMemcpyAsync(&last[1], d_m64 + np - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-        PP_HIP(hipStreamSynchronize(ctx->stream));
-        tot = last[0] + last[1];
+        // the records in all: the sum of m, from the histogram (no read-back)
+        for (int x = 0; x < nb; x++) tot += (int64_t)x * hist[x];
     }
     PP_CHECK(ctx->d_res.grow((size_t)std::max<int64_t>(tot, 1) * PR * sizeof(float) + 4096));
     const size_t nrec = (size_t)std::max<int64_t>(tot, 1);
