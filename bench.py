@@ -146,14 +146,19 @@ def cpu_baseline(cfg, tile, b, ct, seconds: float, runs: int = 5):
 def read_traffic(path, head, kernel):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by the
-    same counter on a copy kernel of known size), or None."""
+    same counter on a copy kernel of known size), and the launch parameters
+    the profiled command pinned (--tune), or (None, None)."""
     if head != "persistent" or not os.path.exists(path):
-        return None
+        return None, None
+    traffic, tune = None, None
     with open(path) as f:
         for line in f:
             if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
-                return float(line.split("=", 1)[1].split()[0])
-    return None
+                traffic = float(line.split("=", 1)[1].split()[0])
+            elif line.startswith("command (FETCH_SIZE)") and "--tune " in line:
+                kv = line.split("--tune ", 1)[1].split()[0]
+                tune = {k: int(v) for k, v in (x.split("=") for x in kv.split(","))}
+    return traffic, tune
 
 
 def serve_stream(args, ct, tuned):
@@ -378,7 +383,12 @@ def main():
         out["plain_kernel_ms"] = float(np.mean(pk)) * 1e3
         out["plain_peaks_per_s"] = npk / float(np.mean(pw))
         out["speedup_vs_plain_kernel"] = float(np.mean(pk)) / k_s
-    out["roofline"]["traffic"] = read_traffic(args.traffic_file, head, dom_name)
+    traffic, traffic_tune = read_traffic(args.traffic_file, head, dom_name)
+    out["roofline"]["traffic"] = traffic
+    if traffic is not None:
+        # the profiled run's launch parameters; the traffic is that run's
+        out["roofline"]["traffic_launch_parameters"] = traffic_tune
+        out["roofline"]["traffic_same_launch_parameters"] = traffic_tune == tuned
     if args.serve_tiles > 0:
         out["serving"] = serve_stream(args, ct, tuned)
         log(f"[bench] serving: {out['serving']}")
