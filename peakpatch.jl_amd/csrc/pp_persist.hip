@@ -1,36 +1,39 @@
 // pp_persist.hip -- batched persistent form of the shell analysis.
 //
-// Pipeline for one pp_analyse_peaks call (on the context stream; after the
-// wave gather the small peaks' gather and lane post run on the context's
-// second stream, beside the wave and lane posts and the finish pass, and are
-// joined back before the call returns):
+// Pipeline for one pp_analyse_peaks call (on the context stream; the packing
+// runs on the context's second stream beside the ordering, and after the wave
+// gather the small peaks' gather, post and finish run there beside the lane
+// post and the finish pass, and are joined back before the call returns):
 //   1. k_pack4: (delta, dT, eta x, eta y) per cell as one float4, with
 //      dT(x,y,z) = delta(z,x,y), so the reference's permuted read
-//      delta[iv3, iv1, iv2] (RadialShell.jl:448) coalesces like the others.
+//      delta[iv3, iv1, iv2] (RadialShell.jl:448) coalesces like the others;
+//      k_pack_q4: (eta z, eta2 x, eta2 y, eta2 z) the same way.
 //   2. peaks are ordered by finalised-shell count m, largest first (already
 //      the case for the peak finder's output, else a radix sort), and every
 //      peak gets a profile of m 24-float shell records (exclusive scan of m).
 //   3. gather.  A peak's sphere is read as octant groups (|di|, |dj|, |dk|):
 //      one lane takes a group and sums its <= 8 mirror images in registers.
-//      * small peaks (m <= tiny_m, at most 64 groups): k_small packs
-//        floor(64 / groups) peaks of equal m into one wave, stages the group
-//        sums in LDS and reduces them per shell in a fixed order.
+//      * small peaks (m <= small_m): k_small reads floor(64 / m) peaks of
+//        equal m cell by cell in row order, hands each cell to the lane of its
+//        (peak, shell) position by a wave shuffle and sums runs of equal shell
+//        by a segmented tree into LDS bins.
 //      * other peaks: k_gather, a persistent work-queue kernel (one wave per
-//        peak, peaks pulled from an atomic counter in m order).  The shells
-//        are processed in chunks of chunk_shells; within a chunk the host
-//        packed the groups (row-major, so lanes read neighbouring x-runs)
-//        into 64-lane batches in which no two groups share a shell, so each
-//        lane deposits into per-shell LDS bins with Neumaier-compensated
-//        FP32 adds, without atomics and in a fixed order.  The chunking keeps
-//        the LDS per wave small, so occupancy does not fall with the radius.
-//   4. Float64 post-process (Fbar, fcrit crossing, walk with strain
-//      eigenvalues and collapse-table lookups, R_TH, Srb, zvir_half):
-//      k_post_wave (one wave per peak, radial sums split over the lanes) for
-//      m > coop_m, k_post_lane (one lane per peak) below.
+//        peak, peaks pulled from per-XCD atomic counters).  The shells are
+//        processed in chunks of chunk_shells; within a chunk the host packed
+//        the groups (row-major, so lanes read neighbouring x-runs) into dense
+//        64-lane batches sorted by shell; the lanes of a run of equal shells
+//        sum by a segmented tree and its first lane deposits into per-shell
+//        LDS bins with Neumaier-compensated FP32 adds, without atomics and in
+//        a fixed order.
+//   4. Float64 post-process (fcrit crossing, inward walk with strain
+//      eigenvalues and collapse-table look-ups, R_TH, Srb, zvir_half):
+//      k_post_wave (one wave per peak) for m > coop_m, k_post_lane (one lane
+//      per peak) below; the lane post's walk steps are evaluated ahead of it
+//      by lane groups (k_walk_group, the walk table).
 //   5. k_finish: one wave per collapsed non-small peak sums eta, eta2 over the
 //      shells inside m0 and lapd over the cells inside R_TH and writes the
-//      mask; small peaks carry those sums in their profile, so their lane
-//      post finishes them.
+//      mask; small peaks carry those sums in their profile, from which
+//      k_small_finish forms them.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
