@@ -53,18 +53,19 @@ def load_table(ctx):
 def time_form(drv, ctx, fields, bdev, mask, form, steps, warmup):
     """Device-resident steps: batch, fields, mask and results stay in HBM."""
     import torch
-    for _ in range(warmup):
-        mask.zero_()
-        torch.cuda.synchronize()
-        drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
-    walls, kern, stages = [], [], []
-    res = None
     # a collection of Python's cyclic garbage collector inside a timed step
     # (tens of ms with torch loaded) would be timed as the step's; collect
-    # now and hold it off until the timed steps are done
+    # now, before the warm-up steps, and hold it off until the timed steps
+    # are done
     gc.collect()
     gc.disable()
+    walls, kern, stages = [], [], []
+    res = None
     try:
+        for _ in range(warmup):
+            mask.zero_()
+            torch.cuda.synchronize()
+            drv.analyse(ctx, fields, bdev, mask=mask, kernel=form, device_out=True)
         for _ in range(steps):
             res = None  # the previous step's result arrays go back to the allocator before this step
             mask.zero_()
