@@ -132,8 +132,9 @@ static int launch_window(pp_ctx *ctx, const void *in, void *out, int n, double b
 // ------------------------------------------------------------- stencil
 // One pass over the field per filter.  A workgroup takes a 64 x 16 tile of
 // inner (x, y) columns and marches along z through kStencilKZ planes: each
-// plane's (64+2) x (16+2) neighbourhood is staged in LDS, every column keeps
-// the 3x3 (x, y) maxima of planes z-1, z, z+1 in registers, and the strict
+// plane's (64+2) x (16+2) neighbourhood is staged in LDS, two planes per
+// barrier (four buffers), every column keeps the 3x3 (x, y) maxima of planes
+// z-1, z, z+1 in registers, and the strict
 // 27-point test "no neighbour above ff" is ff >= max of the 27 values.  A
 // wave is one x-run of 64 cells of one row, so its ballot of accepted cells
 // is one 64-bit word of the row's peak bit mask.  The mask byte is read only
@@ -188,7 +189,7 @@ __global__ void __launch_bounds__(kStencilBlock) k_peak_bits(const float *__rest
                                                              StencilTiles tl, float fcrit, uint64_t *__restrict__ bits)
 {
     constexpr int LX = 64 + 2, LY = kStencilTY + 2, LN = LX * LY, NQ = (LN + kStencilBlock - 1) / kStencilBlock;
-    __shared__ float tile[2][LN];
+    __shared__ float tile[4][LN];  // two pairs of planes: one pair staged while the other is read
     const int t = (int)(blockIdx.x % kStencilXCD) * tl.per_xcd + (int)(blockIdx.x / kStencilXCD);
     if (t >= tl.nbx * tl.nby * tl.nbz) return;  // the whole workgroup leaves before any barrier
     const int tbx = t % tl.nbx, tby = (t / tl.nbx) % tl.nby, tbz = t / (tl.nbx * tl.nby);
@@ -233,38 +234,46 @@ __global__ void __launch_bounds__(kStencilBlock) k_peak_bits(const float *__rest
         return m;
     };
     const long long word_row = (long long)g.words * g.in2;
-    float ca, cb, na_c, nb_c;
+    float ca, cb, c1a, c1b, c2a, c2b;
+    // planes z0-1 and z0 to buffers 2, 3; planes z0+1, z0+2 in flight
     load(z0 - 1, ra);
     load(z0, rb);
-    store(ra, 0);
+    store(ra, 2);
+    store(rb, 3);
     load(z0 + 1, ra);
-    __syncthreads();
-    float pa = max9(0, wy, ca), pb = max9(0, wy + kStencilTY / 2, cb);
-    store(rb, 1);
     if (z0 + 2 <= z1) load(z0 + 2, rb);
     __syncthreads();
-    float qa = max9(1, wy, ca), qb = max9(1, wy + kStencilTY / 2, cb);
-    // plane z+1 sits in r (staged at iteration z); r then fetches plane z+3
-    auto step = [&](int z, float *r) {
-        const int buf = (z - z0) & 1;
-        store(r, buf);
-        if (z + 3 <= z1) load(z + 3, r);
-        __syncthreads();
-        const float ma = max9(buf, wy, na_c), mb = max9(buf, wy + kStencilTY / 2, nb_c);
-        bool ka = ina && ca >= fcrit && !(ca < fmaxf(fmaxf(pa, qa), ma));
-        bool kb = inb && cb >= fcrit && !(cb < fmaxf(fmaxf(pb, qb), mb));
+    float pa = max9(2, wy, ca), pb = max9(2, wy + kStencilTY / 2, cb);
+    float qa = max9(3, wy, ca), qb = max9(3, wy + kStencilTY / 2, cb);
+    auto test = [&](int z, float c_a, float c_b, float mxa, float mxb) {
+        bool ka = ina && c_a >= fcrit && !(c_a < mxa);
+        bool kb = inb && c_b >= fcrit && !(c_b < mxb);
         if (ka) ka = mask[x + (long long)ya * g.n1 + (long long)z * s12] != 1;
         if (kb) kb = mask[x + (long long)yb * g.n1 + (long long)z * s12] != 1;
         const unsigned long long wa = __ballot(ka), wb = __ballot(kb);
         const long long base = (long long)(z - g.nbuff) * word_row + tbx;
         if (lane == 0 && ya < g.nbuff + g.in2) bits[base + (long long)(ya - g.nbuff) * g.words] = wa;
         if (lane == 0 && yb < g.nbuff + g.in2) bits[base + (long long)(yb - g.nbuff) * g.words] = wb;
-        pa = qa; qa = ma; ca = na_c;
-        pb = qb; qb = mb; cb = nb_c;
     };
-    for (int z = z0; z < z1; z += 2) {
-        step(z, ra);
-        if (z + 1 < z1) step(z + 1, rb);
+    // two planes per barrier: planes z+1, z+2 (in ra, rb) go to one pair of
+    // buffers while z+3, z+4 are fetched; the pair written here was last
+    // read two barriers ago
+    for (int z = z0, k = 0; z < z1; z += 2, k ^= 1) {
+        const int b1 = 2 * k, b2 = 2 * k + 1;
+        store(ra, b1);
+        if (z + 2 <= z1) store(rb, b2);
+        if (z + 3 <= z1) load(z + 3, ra);
+        if (z + 4 <= z1) load(z + 4, rb);
+        __syncthreads();
+        const float m1a = max9(b1, wy, c1a), m1b = max9(b1, wy + kStencilTY / 2, c1b);
+        test(z, ca, cb, fmaxf(fmaxf(pa, qa), m1a), fmaxf(fmaxf(pb, qb), m1b));
+        if (z + 1 < z1) {
+            const float m2a = max9(b2, wy, c2a), m2b = max9(b2, wy + kStencilTY / 2, c2b);
+            test(z + 1, c1a, c1b, fmaxf(fmaxf(qa, m1a), m2a), fmaxf(fmaxf(qb, m1b), m2b));
+            pa = m1a; pb = m1b;
+            qa = m2a; qb = m2b;
+            ca = c2a; cb = c2b;
+        }
     }
 }
 
