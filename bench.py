@@ -144,22 +144,31 @@ def cpu_baseline(cfg, tile, b, ct, seconds: float, runs: int = 5):
                          "sample": f"every {stride1}th peak ({len(sel1)}), {t1:.1f} s"}}
 
 
-def read_traffic(path, head, kernel):
+def read_traffic(paths, head, kernel, tuned):
     """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    rocprofv3 --pmc summaries (FETCH_SIZE + WRITE_SIZE, each scaled by the
     same counter on a copy kernel of known size), and the launch parameters
-    the profiled command pinned (--tune), or (None, None)."""
-    if head != "persistent" or not os.path.exists(path):
+    the profiled command pinned (--tune): the summary profiled at the launch
+    parameters this run chose if there is one, else the first; or (None, None)."""
+    if head != "persistent":
         return None, None
-    traffic, tune = None, None
-    with open(path) as f:
-        for line in f:
-            if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
-                traffic = float(line.split("=", 1)[1].split()[0])
-            elif line.startswith("command (FETCH_SIZE)") and "--tune " in line:
-                kv = line.split("--tune ", 1)[1].split()[0]
-                tune = {k: int(v) for k, v in (x.split("=") for x in kv.split(","))}
-    return traffic, tune
+    found = []
+    for path in paths:
+        if not os.path.exists(path):
+            continue
+        traffic, tune = None, None
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
+                    traffic = float(line.split("=", 1)[1].split()[0])
+                elif line.startswith("command (FETCH_SIZE)") and "--tune " in line:
+                    kv = line.split("--tune ", 1)[1].split()[0]
+                    tune = {k: int(v) for k, v in (x.split("=") for x in kv.split(","))}
+        if traffic is not None:
+            found.append((traffic, tune))
+    if not found:
+        return None, None
+    return next((x for x in found if x[1] == tuned), found[0])
 
 
 def serve_stream(args, ct, tuned):
@@ -253,8 +262,11 @@ def main():
                     help="Poisson arrival rate in tiles/s (overrides --serve-load when > 0)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream (a test of it)")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_gather_c3_pmc.txt"),
-                    help="rocprofv3 --pmc summary whose 'traffic_bytes_per_launch' fills roofline.traffic")
+    ap.add_argument("--traffic-file", nargs="+",
+                    default=[os.path.join(ROOT, "profiles", f) for f in ("k_gather_c3_pmc.txt",
+                                                                         "k_gather_c3_pmc_gwc24.txt")],
+                    help="rocprofv3 --pmc summaries whose 'traffic_bytes_per_launch' fills roofline.traffic "
+                         "(the one profiled at this run's launch parameters, else the first)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -384,7 +396,7 @@ def main():
         out["plain_kernel_ms"] = float(np.mean(pk)) * 1e3
         out["plain_peaks_per_s"] = npk / float(np.mean(pw))
         out["speedup_vs_plain_kernel"] = float(np.mean(pk)) / k_s
-    traffic, traffic_tune = read_traffic(args.traffic_file, head, dom_name)
+    traffic, traffic_tune = read_traffic(args.traffic_file, head, dom_name, tuned)
     out["roofline"]["traffic"] = traffic
     if traffic is not None:
         # the profiled run's launch parameters; the traffic is that run's
