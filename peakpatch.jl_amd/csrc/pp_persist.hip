@@ -692,10 +692,12 @@ __device__ __forceinline__ AosProfile<FULL> profile_at(ProfBuf p, int64_t rb, co
 // launch grids (measured): workgroups per CU of k_small (grid-stride), waves
 // per CU of the wave post, the finish pass and the lane posts (grid-stride:
 // finer rounds balance better); the lane post asks for one wave per SIMD as
-// its register budget
+// its register budget.  The finish pass at c3 (whole call, kernel time): 32
+// waves per CU 15.55 ms, 64 15.33, 128 15.14, 256 15.05, 512 15.06, 4096
+// (about one peak per wave) 15.10
 constexpr int kSmallGridCU = 32;
 constexpr int kWaveGridCU = 16;
-constexpr int kFinishGridCU = 64;
+constexpr int kFinishGridCU = 256;
 constexpr int kLaneGridCU = 64;
 constexpr int kWalkGridCU = 16;  // 256-lane workgroups per CU of the walk table (grid-stride over the peaks)
 
